@@ -1,0 +1,524 @@
+// capi.hip -- host runtime and C-ABI of liboptphot_hip.so (include/optphot_hip.h).
+//
+// One context per process and device: a non-blocking HIP stream at the
+// default priority, timing events, and grow-only device buffers that are
+// reused across calls (their first allocation is reported as alloc_ms, as the
+// reference reports its per-call cudaMalloc, cuda_util.cuh:88-96). Every
+// entry point validates its arguments before touching the GPU, runs
+// synchronously on the context's stream, and converts exceptions into a
+// return code plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/optphot_hip.h"
+#include "kernels.hpp"
+
+namespace {
+
+using namespace ophip;
+
+static_assert(sizeof(optphot_sim_params) == 72, "SimParams layout (params.hpp:40-63)");
+static_assert(sizeof(optphot_tally_config) == 12, "TallyConfig layout (transport.hpp:164-168)");
+
+thread_local std::string g_last_error;
+
+struct InvalidArgument : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+
+void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess) {
+    throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " at " + file +
+                             ":" + std::to_string(line) + " (" + what + ")");
+  }
+}
+#define HIPCHK(call) hip_check((call), #call, __FILE__, __LINE__)
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  // Returns true if it had to (re)allocate.
+  bool ensure(size_t want) {
+    if (want <= bytes && ptr) return false;
+    if (ptr) HIPCHK(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    want = std::max<size_t>(want, 256);
+    HIPCHK(hipMalloc(&ptr, want));
+    bytes = want;
+    return true;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+struct Context {
+  std::mutex mu;
+  bool ready = false;
+  int device = 0;
+  int cu_count = 0;
+  size_t max_lds = 0;
+  std::string name;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  DevBuf records[8];  // fate, nb, ns, t, x, y, z, path
+  DevBuf partials, scratch, tally;
+
+  void init() {
+    if (ready) return;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      throw std::runtime_error("no HIP device available");
+    HIPCHK(hipGetDevice(&device));
+    hipDeviceProp_t prop{};
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    cu_count = prop.multiProcessorCount;
+    max_lds = prop.sharedMemPerBlock;
+    name = std::string(prop.name) + " (" + prop.gcnArchName + ", " + std::to_string(cu_count) +
+           " CUs)";
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    ready = true;
+  }
+
+  float elapsed(int a, int b) {
+    HIPCHK(hipEventSynchronize(ev[b]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+    return ms;
+  }
+};
+
+Context& ctx() {
+  static Context c;
+  return c;
+}
+
+// ---- parameter conversion -------------------------------------------------
+
+DevParams to_dev(const optphot_sim_params& p) {
+  DevParams d{};
+  d.hx = p.hx;
+  d.hy = p.hy;
+  d.hz = p.hz;
+  d.inv_speed = p.n_scint / 299.792458f;  // transport.hpp:74
+  d.abs_length = p.abs_length;
+  d.scat_length = p.scat_length;
+  d.n_scint = p.n_scint;
+  d.n_det = p.n_det;
+  d.n_out = p.n_out;
+  const float eta_det = p.n_scint / p.n_det;  // fresnel.hpp:37-38
+  const float eta_out = p.n_scint / p.n_out;
+  d.eta2_det = eta_det * eta_det;
+  d.eta2_out = eta_out * eta_out;
+  d.det_matched = p.n_scint == p.n_det;
+  d.out_matched = p.n_scint == p.n_out;
+  d.reflectivity = p.reflectivity;
+  d.src_x = p.src_x;
+  d.src_y = p.src_y;
+  d.src_z = p.src_z;
+  d.tau = p.tau;
+  d.surface = p.surface;
+  d.source = p.source;
+  d.max_steps = p.max_steps;
+  return d;
+}
+
+PhiloxKey key_of(const optphot_sim_params& p) {
+  return PhiloxKey{static_cast<uint32_t>(p.seed), static_cast<uint32_t>(p.seed >> 32)};
+}
+
+void validate_params(const optphot_sim_params* p) {
+  if (!p) throw InvalidArgument("params is NULL");
+  if (p->surface > 3u) throw InvalidArgument("surface must be 0..3 (polished|black|specular|lambertian)");
+  if (p->source > 1u) throw InvalidArgument("source must be 0 (point) or 1 (volume)");
+  if (!(p->hx > 0.0f && p->hy > 0.0f && p->hz > 0.0f))
+    throw InvalidArgument("box half sizes must be positive");
+}
+
+void validate_block(int block_size, const char* who) {
+  if (block_size <= 0 || block_size % 64 != 0 || block_size > 1024)
+    throw InvalidArgument(std::string(who) + ": block_size must be a multiple of 64 <= 1024");
+}
+
+DevTally to_dev_tally(const optphot_tally_config& c, const optphot_sim_params& p) {
+  if (c.time_bins == 0 || c.xy_bins == 0) throw InvalidArgument("tally bins must be >= 1");
+  if (!(c.time_max > 0.0f)) throw InvalidArgument("time_max must be > 0");
+  if (c.xy_bins > 4096u || c.time_bins > (1u << 24))
+    throw InvalidArgument("tally binning too large");
+  DevTally t{};
+  t.time_bins = c.time_bins;
+  t.xy_bins = c.xy_bins;
+  t.time_scale = static_cast<float>(c.time_bins) / c.time_max;  // transport.hpp:171
+  t.two_hx = 2.0f * p.hx;                                       // transport.hpp:178-179
+  t.two_hy = 2.0f * p.hy;
+  t.words = 5u + c.time_bins + 1u + c.xy_bins * c.xy_bins;
+  return t;
+}
+
+void check_lds(Context& c, const DevTally& t) {
+  const size_t bytes = static_cast<size_t>(t.words) * sizeof(uint32_t);
+  if (bytes > c.max_lds) {
+    throw std::runtime_error("tally histograms need " + std::to_string(bytes) +
+                             " bytes of LDS; reduce xy_bins / time_bins");
+  }
+}
+
+// ---- records helpers --------------------------------------------------------
+
+DevRecords ensure_records(Context& c, uint64_t cap, double& alloc_ms) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
+  bool grew = false;
+  for (int i = 0; i < 8; ++i) grew |= c.records[i].ensure(cap * sz[i]);
+  if (grew) alloc_ms += ms_since(t0);
+  return DevRecords{c.records[0].as<uint8_t>(),  c.records[1].as<uint32_t>(),
+                    c.records[2].as<uint32_t>(), c.records[3].as<float>(),
+                    c.records[4].as<float>(),    c.records[5].as<float>(),
+                    c.records[6].as<float>(),    c.records[7].as<float>()};
+}
+
+void copy_records(Context& c, const optphot_records* out, uint64_t offset, uint64_t m) {
+  void* dst[8] = {out->fate + offset, out->n_boundary + offset, out->n_scatter + offset,
+                  out->t + offset,    out->x + offset,          out->y + offset,
+                  out->z + offset,    out->path + offset};
+  const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
+  for (int i = 0; i < 8; ++i)
+    HIPCHK(hipMemcpyAsync(dst[i], c.records[i].ptr, m * sz[i], hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+}
+
+void validate_records(const optphot_records* out) {
+  if (!out || !out->fate || !out->n_boundary || !out->n_scatter || !out->t || !out->x || !out->y ||
+      !out->z || !out->path)
+    throw InvalidArgument("records: every output array must be non-NULL");
+}
+
+// Scratch words: [0] queue head, [1 .. 16] counters, [17 ..] block sums.
+constexpr size_t kScratchQueue = 0, kScratchCounters = 1, kScratchSums = 1 + kCntSlots;
+
+unsigned long long* ensure_scratch(Context& c, int grid, double& alloc_ms) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (c.scratch.ensure((kScratchSums + 2 * static_cast<size_t>(grid)) * 8)) alloc_ms += ms_since(t0);
+  return c.scratch.as<unsigned long long>();
+}
+
+void read_counters(Context& c, optphot_step_counters* counters) {
+  if (!counters) return;
+  HIPCHK(hipMemcpyAsync(counters->v, c.scratch.as<unsigned long long>() + kScratchCounters,
+                        sizeof(counters->v), hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+}
+
+// ---- the runs ---------------------------------------------------------------
+
+int history_grid_for_tally(Context& c, int block, size_t lds, uint64_t n) {
+  const int per_cu = std::max(1, history_blocks_per_cu(true, block, lds));
+  const uint64_t needed = (n + block - 1) / block;
+  return static_cast<int>(std::max<uint64_t>(
+      1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
+}
+
+void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tally_config& tc,
+                      uint64_t n, uint64_t* tally_out, uint64_t first_id, bool wavefront,
+                      int block, int grid, const WavefrontShape& shape,
+                      optphot_run_timing& tm, optphot_event_stats* stats,
+                      optphot_step_counters* counters) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const DevTally T = to_dev_tally(tc, p);
+  check_lds(c, T);
+  if (n / static_cast<uint64_t>(grid) >= 0xffffffffull)
+    throw std::runtime_error("too many photons for one tally launch; split the run");
+  unsigned long long* scratch = ensure_scratch(c, grid, tm.alloc_ms);
+  const auto ta = std::chrono::steady_clock::now();
+  bool grew = c.partials.ensure(static_cast<size_t>(grid) * T.words * sizeof(uint32_t));
+  grew |= c.tally.ensure((static_cast<size_t>(T.words) + 2) * 8);
+  if (grew) tm.alloc_ms += ms_since(ta);
+
+  LaunchOut out{};
+  out.tally = T;
+  out.partials = c.partials.as<uint32_t>();
+  out.sums = scratch + kScratchSums;
+  out.counters = scratch + kScratchCounters;
+  out.tally_on = true;
+  out.records = false;
+
+  launch_zero_u64(scratch, kScratchSums + 2 * static_cast<size_t>(grid), c.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.ev[0], c.stream));
+  if (wavefront)
+    launch_wavefront(to_dev(p), key_of(p), first_id, n, shape, scratch + kScratchQueue, out,
+                     c.stream);
+  else
+    launch_history(to_dev(p), key_of(p), first_id, n, block, grid, out, c.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.ev[1], c.stream));
+  launch_tally_reduce(T, grid, out.partials, out.sums, c.tally.as<unsigned long long>(), c.stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c.ev[2], c.stream));
+  const float k_ms = c.elapsed(0, 1);
+  tm.kernel_ms = c.elapsed(0, 2);
+
+  const auto td = std::chrono::steady_clock::now();
+  HIPCHK(hipMemcpyAsync(tally_out, c.tally.ptr, (static_cast<size_t>(T.words) + 2) * 8,
+                        hipMemcpyDeviceToHost, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  tm.d2h_ms = ms_since(td);
+  read_counters(c, counters);
+  if (stats) {
+    optphot_step_counters cnt{};
+    read_counters(c, &cnt);
+    stats->launches = 1;
+    stats->useful_lane_steps = cnt.v[kCntUseful];
+    stats->issued_lane_steps = cnt.v[kCntIssued];
+    stats->kernels_ms = k_ms;
+  }
+  tm.h2d_ms = 0.0;  // photons are generated on the device (gpu_backend.cu:151-153)
+  tm.total_ms = ms_since(t0);
+}
+
+WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, bool records,
+                               bool tally, size_t lds, uint64_t n) {
+  optphot_wavefront_tuning t{256, 4, 256, 0};
+  if (tu) {
+    if (tu->block_size) t.block_size = tu->block_size;
+    if (tu->steps_per_refill) t.steps_per_refill = tu->steps_per_refill;
+    if (tu->queue_chunk) t.queue_chunk = tu->queue_chunk;
+    if (tu->blocks_per_cu) t.blocks_per_cu = tu->blocks_per_cu;
+  }
+  validate_block(t.block_size, "wavefront");
+  if (t.queue_chunk < 64 || t.queue_chunk % 64 != 0 || t.queue_chunk > (1u << 24))
+    throw InvalidArgument("wavefront: queue_chunk must be a multiple of 64 in [64, 2^24]");
+  if (t.steps_per_refill == 0)
+    throw InvalidArgument("wavefront: steps_per_refill must be >= 1");
+  const int max_per_cu = std::max(1, wavefront_blocks_per_cu(records, tally, t.block_size, lds));
+  const int per_cu = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, max_per_cu) : max_per_cu;
+  const uint64_t needed = (n + t.block_size - 1) / t.block_size;
+  WavefrontShape s{};
+  s.block = t.block_size;
+  s.grid = static_cast<int>(std::max<uint64_t>(
+      1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
+  s.steps_per_refill = t.steps_per_refill;
+  s.queue_chunk = t.queue_chunk;
+  return s;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return OPTPHOT_OK;
+  } catch (const InvalidArgument& e) {
+    g_last_error = e.what();
+    return OPTPHOT_ERR_INVALID_ARGUMENT;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return std::strstr(e.what(), "no HIP device") ? OPTPHOT_ERR_NO_DEVICE : OPTPHOT_ERR_RUNTIME;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return OPTPHOT_ERR_RUNTIME;
+  }
+}
+
+}  // namespace
+
+// =============================================================================
+// C-ABI
+// =============================================================================
+
+extern "C" {
+
+__attribute__((visibility("default"))) int optphot_gpu_available(void) {
+  int count = 0;
+  return hipGetDeviceCount(&count) == hipSuccess && count > 0 ? 1 : 0;
+}
+
+__attribute__((visibility("default"))) const char* optphot_last_error(void) {
+  return g_last_error.c_str();
+}
+
+__attribute__((visibility("default"))) int optphot_gpu_device_name(char* buf, size_t len) {
+  return guarded([&] {
+    if (!buf || len == 0) throw InvalidArgument("buffer is NULL");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    std::snprintf(buf, len, "%s", c.name.c_str());
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_device_info(char* buf, size_t len) {
+  return guarded([&] {
+    if (!buf || len == 0) throw InvalidArgument("buffer is NULL");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    std::string s = "name=" + c.name + "\ncu_count=" + std::to_string(c.cu_count) +
+                    "\nmax_lds_per_block=" + std::to_string(c.max_lds) + "\n";
+    for (int b : {64, 128, 256, 512, 1024}) {
+      s += "wavefront_tally_blocks_per_cu_" + std::to_string(b) + "=" +
+           std::to_string(wavefront_blocks_per_cu(false, true, b, 17432)) + "\n";
+      s += "history_records_blocks_per_cu_" + std::to_string(b) + "=" +
+           std::to_string(history_blocks_per_cu(false, b, 0)) + "\n";
+    }
+    std::snprintf(buf, len, "%s", s.c_str());
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_records(
+    const optphot_sim_params* p, uint64_t n, const optphot_records* out, int block_size,
+    uint64_t first_id, uint64_t chunk, optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    validate_block(block_size, "history-based backend");
+    if (n > 0) validate_records(out);
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(n, chunk ? chunk : 1));
+    const DevRecords rec = ensure_records(c, cap, tm.alloc_ms);
+    unsigned long long* scratch = ensure_scratch(c, 1, tm.alloc_ms);
+    LaunchOut lo{};
+    lo.rec = rec;
+    lo.records = true;
+    lo.counters = scratch + kScratchCounters;
+    const DevParams P = to_dev(*p);
+    for (uint64_t done = 0; done < n; done += cap) {
+      const uint64_t m = std::min(cap, n - done);
+      const int grid = static_cast<int>((m + block_size - 1) / block_size);
+      HIPCHK(hipEventRecord(c.ev[0], c.stream));
+      launch_history(P, key_of(*p), first_id + done, m, block_size, grid, lo, c.stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c.ev[1], c.stream));
+      tm.kernel_ms += c.elapsed(0, 1);
+      const auto td = std::chrono::steady_clock::now();
+      copy_records(c, out, done, m);
+      tm.d2h_ms += ms_since(td);
+    }
+    tm.total_ms = ms_since(t0);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_tally(
+    const optphot_sim_params* p, const optphot_tally_config* tc, uint64_t n, uint64_t* tally,
+    int block_size, uint64_t first_id, optphot_run_timing* timing) {
+  return optphot_run_gpu_tally_counters(p, tc, n, tally, block_size, first_id, nullptr, timing);
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_tally_counters(
+    const optphot_sim_params* p, const optphot_tally_config* tc, uint64_t n, uint64_t* tally,
+    int block_size, uint64_t first_id, optphot_step_counters* counters,
+    optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    validate_block(block_size, "history-based backend");
+    if (!tc || !tally) throw InvalidArgument("tally config / output is NULL");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    const DevTally T = to_dev_tally(*tc, *p);
+    const size_t lds = static_cast<size_t>(T.words) * 4;
+    check_lds(c, T);
+    const int grid = history_grid_for_tally(c, block_size, lds, n);
+    optphot_run_timing tm{};
+    run_tally_common(c, *p, *tc, n, tally, first_id, false, block_size, grid, WavefrontShape{},
+                     tm, nullptr, counters);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_tally_wavefront(
+    const optphot_sim_params* p, const optphot_tally_config* tc, uint64_t n, uint64_t* tally,
+    uint64_t first_id, const optphot_wavefront_tuning* tuning, optphot_event_stats* stats,
+    optphot_step_counters* counters, optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    if (!tc || !tally) throw InvalidArgument("tally config / output is NULL");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    const DevTally T = to_dev_tally(*tc, *p);
+    const size_t lds = static_cast<size_t>(T.words) * 4;
+    check_lds(c, T);
+    const WavefrontShape s = wavefront_shape(c, tuning, false, true, lds, n);
+    optphot_run_timing tm{};
+    run_tally_common(c, *p, *tc, n, tally, first_id, true, s.block, s.grid, s, tm, stats,
+                     counters);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_records_event(
+    const optphot_sim_params* p, uint64_t n, const optphot_records* out,
+    uint32_t steps_per_launch, int block_size, uint64_t first_id, uint64_t chunk,
+    optphot_event_stats* stats, optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    validate_block(block_size, "event-based backend");
+    if (steps_per_launch == 0)
+      throw InvalidArgument("event-based backend: steps_per_launch must be >= 1");
+    if (n > 0) validate_records(out);
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    optphot_run_timing tm{};
+    optphot_event_stats st{};
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t cap = std::max<uint64_t>(
+        1, std::min<uint64_t>({n, chunk ? chunk : 1, uint64_t{1} << 31}));
+    const DevRecords rec = ensure_records(c, cap, tm.alloc_ms);
+    optphot_wavefront_tuning tu{block_size, steps_per_launch, 256, 0};
+    const WavefrontShape s = wavefront_shape(c, &tu, true, false, 0, cap);
+    unsigned long long* scratch = ensure_scratch(c, s.grid, tm.alloc_ms);
+    LaunchOut lo{};
+    lo.rec = rec;
+    lo.records = true;
+    lo.counters = scratch + kScratchCounters;
+    const DevParams P = to_dev(*p);
+    launch_zero_u64(scratch, kScratchSums, c.stream);  // queue + counters
+    HIPCHK(hipGetLastError());
+    for (uint64_t done = 0; done < n; done += cap) {
+      const uint64_t m = std::min(cap, n - done);
+      const WavefrontShape sm = wavefront_shape(c, &tu, true, false, 0, m);
+      launch_zero_u64(scratch + kScratchQueue, 1, c.stream);
+      HIPCHK(hipEventRecord(c.ev[0], c.stream));
+      launch_wavefront(P, key_of(*p), first_id + done, m, sm, scratch + kScratchQueue, lo,
+                       c.stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c.ev[1], c.stream));
+      const float k = c.elapsed(0, 1);
+      tm.kernel_ms += k;
+      st.kernels_ms += k;
+      ++st.launches;
+      const auto td = std::chrono::steady_clock::now();
+      copy_records(c, out, done, m);
+      tm.d2h_ms += ms_since(td);
+    }
+    optphot_step_counters cnt{};
+    read_counters(c, &cnt);
+    st.useful_lane_steps = cnt.v[kCntUseful];
+    st.issued_lane_steps = cnt.v[kCntIssued];
+    tm.total_ms = ms_since(t0);
+    if (stats) *stats = st;
+    if (timing) *timing = tm;
+  });
+}
+
+}  // extern "C"

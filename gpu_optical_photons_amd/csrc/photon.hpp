@@ -1,0 +1,425 @@
+// photon.hpp -- device-side optical-photon physics for gfx950 (wave64).
+//
+// One lane owns one photon. The arithmetic of every operation follows the
+// reference's physics (include/optphot/{philox,pmath,vec3,sampling,fresnel,
+// geometry,transport}.hpp) in the same evaluation order, because only then do
+// the correctly rounded f32 operations produce the same bits as the CPU path.
+// The code is compiled with -ffp-contract=off and IEEE division / square root.
+//
+// What is organised differently from the reference, for wave64 SIMT:
+//  * the step is mostly branch-free: every lane computes the travelled
+//    distance d = min(absorb, scatter, wall) and advances with one update;
+//    Fresnel (readout or polished face) is evaluated once for all lanes that
+//    need it, and one rotate_about() serves both Rayleigh scattering and
+//    Lambertian reflection, so mixed waves do not execute those bodies twice;
+//  * specular reflection on an axis-aligned face is the exact negation of one
+//    direction component (d - 2 (d.n) n rounds to exactly that);
+//  * the Philox stream keeps an 8-word window (two blocks) that is topped up
+//    once, at a wave-uniform point at the top of the step, so that the <= 5
+//    words a step consumes never trigger a Philox call inside a divergent
+//    branch. Each draw then reads the window at a compile-time offset.
+// Draw order, decision inequalities, tie rules and fate codes are exactly the
+// reference's (transport.hpp:55-145), so photon i's history is the same.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ophip {
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr int kReadoutFace = 5;
+
+enum : uint32_t { kPolished = 0, kBlack = 1, kSpecular = 2, kLambertian = 3 };
+enum : uint32_t { kPoint = 0, kVolume = 1 };
+enum : uint32_t {
+  kDetected = 0,
+  kAbsorbedBulk = 1,
+  kAbsorbedSurface = 2,
+  kEscaped = 3,
+  kMaxSteps = 4
+};
+
+// Per-run constants in the form the step uses. Derived fields are computed on
+// the host with the same single IEEE operation the reference performs inside
+// the loop (inv_speed = n_scint / c, transport.hpp:74; eta = n1 / n2 and
+// eta * eta, fresnel.hpp:37-38), so hoisting them changes no bit.
+struct DevParams {
+  float hx, hy, hz;
+  float inv_speed;
+  float abs_length, scat_length;  // +inf disables the process
+  float n_scint, n_det, n_out;
+  float eta2_det, eta2_out;       // (n_scint / n_det)^2, (n_scint / n_out)^2
+  float reflectivity;
+  float src_x, src_y, src_z;
+  float tau;
+  uint32_t det_matched, out_matched;  // n_scint == n_det, n_scint == n_out
+  uint32_t surface, source, max_steps;
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC11; the reference's philox.hpp:37-76).
+// Counter {c0, 0, c2, c3}; the second word is always 0 in this path.
+
+struct PhiloxKey {
+  uint32_t k0, k1;
+};
+
+__device__ __forceinline__ void philox10(uint32_t c0, uint32_t c2, uint32_t c3, PhiloxKey key,
+                                         uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
+  uint32_t x0 = c0, x1 = 0u, x2 = c2, x3 = c3;
+  uint32_t k0 = key.k0, k1 = key.k1;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = static_cast<uint64_t>(x0) * 0xD2511F53u;
+    const uint64_t p1 = static_cast<uint64_t>(x2) * 0xCD9E8D57u;
+    const uint32_t y0 = static_cast<uint32_t>(p1 >> 32) ^ x1 ^ k0;
+    const uint32_t y2 = static_cast<uint32_t>(p0 >> 32) ^ x3 ^ k1;
+    x1 = static_cast<uint32_t>(p1);
+    x3 = static_cast<uint32_t>(p0);
+    x0 = y0;
+    x2 = y2;
+    k0 += 0x9E3779B9u;  // Weyl key schedule
+    k1 += 0xBB67AE85u;
+  }
+  o0 = x0;
+  o1 = x1;
+  o2 = x2;
+  o3 = x3;
+}
+
+// Top 24 bits -> k * 2^-24 in (0, 1] (philox.hpp:82-84).
+__device__ __forceinline__ float to_unit(uint32_t bits) {
+  return static_cast<float>((bits >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// A photon's stream: words [4*(blk-2), 4*blk) are held in w0..w7, `pos` is
+// the next unread word inside that window. Refill keeps pos <= 3, so at least
+// five words are readable without generating.
+struct Stream {
+  uint32_t w0, w1, w2, w3, w4, w5, w6, w7;
+  uint32_t pos, blk;
+  uint32_t id_lo, id_hi;
+
+  __device__ __forceinline__ void start(uint64_t photon_id, PhiloxKey key, uint32_t draws) {
+    id_lo = static_cast<uint32_t>(photon_id);
+    id_hi = static_cast<uint32_t>(photon_id >> 32);
+    const uint32_t b = draws >> 2;
+    philox10(b, id_lo, id_hi, key, w0, w1, w2, w3);
+    philox10(b + 1u, id_lo, id_hi, key, w4, w5, w6, w7);
+    blk = b + 2u;
+    pos = draws & 3u;
+  }
+
+  __device__ __forceinline__ void refill(PhiloxKey key) {
+    while (pos >= 4u) {
+      w0 = w4;
+      w1 = w5;
+      w2 = w6;
+      w3 = w7;
+      philox10(blk, id_lo, id_hi, key, w4, w5, w6, w7);
+      ++blk;
+      pos -= 4u;
+    }
+  }
+
+  // Word at window offset pos + k, k in [0, 4] (pos in [0, 3]).
+  template <int K>
+  __device__ __forceinline__ uint32_t peek() const {
+    const uint32_t i = pos + K;
+    const uint32_t a = (i & 1u) ? w1 : w0;
+    const uint32_t b = (i & 1u) ? w3 : w2;
+    const uint32_t c = (i & 1u) ? w5 : w4;
+    const uint32_t d = (i & 1u) ? w7 : w6;
+    const uint32_t lo = (i & 2u) ? b : a;
+    const uint32_t hi = (i & 2u) ? d : c;
+    return (i & 4u) ? hi : lo;
+  }
+
+  __device__ __forceinline__ uint32_t draws() const { return 4u * (blk - 2u) + pos; }
+};
+
+// ---------------------------------------------------------------------------
+// Transcendentals from correctly rounded operations only (the reference's
+// portable math, pmath.hpp:62-130): bit-identical to the CPU build with
+// OPTPHOT_PORTABLE_MATH.
+
+__device__ __forceinline__ float p_log(float x) {
+  const uint32_t bits = __float_as_uint(x);
+  int e = static_cast<int>((bits >> 23) & 0xffu) - 127;
+  float m = __uint_as_float((bits & 0x007fffffu) | 0x3f800000u);
+  if (m > 1.41421356f) {
+    m = 0.5f * m;
+    ++e;
+  }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  const float poly =
+      1.0f + s2 * (1.0f / 3.0f +
+                   s2 * (1.0f / 5.0f + s2 * (1.0f / 7.0f + s2 * (1.0f / 9.0f + s2 * (1.0f / 11.0f)))));
+  const float fe = static_cast<float>(e);
+  return fe * 6.9313812256e-01f + (2.0f * s * poly + fe * 9.0580006145e-06f);
+}
+
+__device__ __forceinline__ void p_sincos_turns(float u, float& sn, float& cs) {
+  const float v = 4.0f * u;
+  float q = __builtin_floorf(v);
+  float r = v - q;
+  if (r > 0.5f) {
+    r -= 1.0f;
+    q += 1.0f;
+  }
+  const float x = r * 1.57079632679489662f;
+  const float x2 = x * x;
+  const float sp =
+      x * (1.0f + x2 * (-1.0f / 6.0f +
+                        x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f + x2 * (1.0f / 362880.0f)))));
+  const float cp =
+      1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f +
+                                 x2 * (-1.0f / 720.0f +
+                                       x2 * (1.0f / 40320.0f + x2 * (-1.0f / 3628800.0f)))));
+  const int quadrant = static_cast<int>(q) & 3;
+  // quadrant rotation as selects: (s, c) = (sp, cp), (cp, -sp), (-sp, -cp), (-cp, sp)
+  const bool odd = quadrant & 1;
+  const float a = odd ? cp : sp;
+  const float b = odd ? sp : cp;
+  sn = (quadrant & 2) ? -a : a;
+  cs = (quadrant == 1 || quadrant == 2) ? -b : b;
+}
+
+__device__ __forceinline__ float p_cbrt(float x) {
+  float y = __uint_as_float(__float_as_uint(x) / 3u + 0x2a514067u);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) y = y - (y * y * y - x) / (3.0f * y * y);
+  return y;
+}
+
+__device__ __forceinline__ float f_sqrt(float x) { return __builtin_sqrtf(x); }
+
+// ---------------------------------------------------------------------------
+// Vector helpers (vec3.hpp): only what the step needs, same operation order.
+
+struct V3 {
+  float x, y, z;
+};
+
+// rotate_about (vec3.hpp:35-53): Duff et al. basis around `n`, then
+// normalize((st cp) t1 + (st sp) t2 + ct n) with normalize(a) = (1/|a|) a.
+__device__ __forceinline__ V3 rotate_about(V3 n, float cos_theta, float phi_turns) {
+  const float sign = __builtin_copysignf(1.0f, n.z);
+  const float a = -1.0f / (sign + n.z);
+  const float b = n.x * n.y * a;
+  const V3 t1{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+  const V3 t2{b, sign + n.y * n.y * a, -n.y};
+  const float st = f_sqrt(__builtin_fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
+  float sp, cp;
+  p_sincos_turns(phi_turns, sp, cp);
+  const float c1 = st * cp, c2 = st * sp;
+  const V3 d{(c1 * t1.x + c2 * t2.x) + cos_theta * n.x, (c1 * t1.y + c2 * t2.y) + cos_theta * n.y,
+             (c1 * t1.z + c2 * t2.z) + cos_theta * n.z};
+  const float inv = 1.0f / f_sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+  return V3{inv * d.x, inv * d.y, inv * d.z};
+}
+
+// Cardano root of the Rayleigh CDF (sampling.hpp:39-44).
+__device__ __forceinline__ float rayleigh_cos(float u) {
+  const float q = 4.0f * u - 2.0f;
+  const float s = f_sqrt(q * q + 1.0f);
+  const float a = (q >= 0.0f) ? p_cbrt(q + s) : 1.0f / p_cbrt(s - q);
+  return __builtin_fminf(1.0f, __builtin_fmaxf(-1.0f, a - 1.0f / a));
+}
+
+// ---------------------------------------------------------------------------
+// Photon state and the transport step.
+
+struct Photon {
+  V3 p, d;
+  float t, path;
+  uint32_t nb, ns;
+};
+
+// Per-step classification, exposed so kernels can count branch events.
+struct StepInfo {
+  bool absorb, scatter, readout, polished, black, refl, survived, lambert_reflect;
+};
+
+// Emission (transport.hpp:55-67): [x, y, z if volume], cos, phi, time.
+// The stream window must hold 8 fresh words (pos == 0 after start()).
+__device__ __forceinline__ void emit(const DevParams& P, Stream& rng, Photon& ph) {
+  uint32_t o = 0;
+  if (P.source == kVolume) {
+    const float u1 = to_unit(rng.peek<0>());
+    const float u2 = to_unit(rng.peek<1>());
+    const float u3 = to_unit(rng.peek<2>());
+    ph.p = V3{P.hx * (2.0f * u1 - 1.0f), P.hy * (2.0f * u2 - 1.0f), P.hz * (2.0f * u3 - 1.0f)};
+    o = 3;
+  } else {
+    ph.p = V3{P.src_x, P.src_y, P.src_z};
+  }
+  // The window holds words 0..7 and o <= 3, so the offsets o..o+2 are read
+  // through peek with pos advanced first.
+  rng.pos += o;
+  const float ua = to_unit(rng.peek<0>());
+  const float ub = to_unit(rng.peek<1>());
+  const float ut = to_unit(rng.peek<2>());
+  rng.pos += 3u;
+  const float cos_t = 2.0f * ua - 1.0f;
+  const float sin_t = f_sqrt(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));
+  float sp, cp;
+  p_sincos_turns(ub, sp, cp);
+  ph.d = V3{sin_t * cp, sin_t * sp, cos_t};
+  ph.t = P.tau > 0.0f ? -P.tau * p_log(ut) : 0.0f;
+  ph.path = 0.0f;
+  ph.nb = 0u;
+  ph.ns = 0u;
+}
+
+// One transport step (transport.hpp:71-145). Returns true while the photon
+// is alive; otherwise *fate is set. The caller must have called
+// rng.refill() so that pos <= 3.
+__device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph, uint32_t& fate,
+                                     StepInfo& info) {
+  // --- two exponential distances, always drawn (transport.hpp:76-77) -------
+  const float u_abs = to_unit(rng.peek<0>());
+  const float u_sca = to_unit(rng.peek<1>());
+  float s_abs = kInf, s_sca = kInf;
+  if (P.abs_length != kInf) s_abs = -P.abs_length * p_log(u_abs);
+  if (P.scat_length != kInf) s_sca = -P.scat_length * p_log(u_sca);
+
+  // --- distance to the wall approached, ties to the lower axis (geometry.hpp:25-46)
+  float dist = kInf;
+  int face = -1;
+  if (ph.d.x != 0.0f) {
+    const bool pos = ph.d.x > 0.0f;
+    const float tt = ((pos ? P.hx : -P.hx) - ph.p.x) / ph.d.x;
+    if (tt < dist) {
+      dist = tt;
+      face = pos ? 1 : 0;
+    }
+  }
+  if (ph.d.y != 0.0f) {
+    const bool pos = ph.d.y > 0.0f;
+    const float tt = ((pos ? P.hy : -P.hy) - ph.p.y) / ph.d.y;
+    if (tt < dist) {
+      dist = tt;
+      face = pos ? 3 : 2;
+    }
+  }
+  if (ph.d.z != 0.0f) {
+    const bool pos = ph.d.z > 0.0f;
+    const float tt = ((pos ? P.hz : -P.hz) - ph.p.z) / ph.d.z;
+    if (tt < dist) {
+      dist = tt;
+      face = pos ? 5 : 4;
+    }
+  }
+  dist = __builtin_fmaxf(dist, 0.0f);
+
+  // --- which event wins, and one shared advance ------------------------------
+  const bool absorb = s_abs < dist && s_abs <= s_sca;
+  const bool scatter = !absorb && s_sca < dist;
+  const bool boundary = !absorb && !scatter;
+  const float go = absorb ? s_abs : (scatter ? s_sca : dist);
+  ph.p = V3{ph.p.x + go * ph.d.x, ph.p.y + go * ph.d.y, ph.p.z + go * ph.d.z};
+  ph.t += go * P.inv_speed;
+  ph.path += go;
+
+  const int axis = face >> 1;
+  const bool plus = face & 1;
+  if (boundary) {  // snap onto the face plane, clamp the others (geometry.hpp:57-63)
+    const float s = plus ? 1.0f : -1.0f;
+    ph.p.x = axis == 0 ? s * P.hx : __builtin_fminf(__builtin_fmaxf(ph.p.x, -P.hx), P.hx);
+    ph.p.y = axis == 1 ? s * P.hy : __builtin_fminf(__builtin_fmaxf(ph.p.y, -P.hy), P.hy);
+    ph.p.z = axis == 2 ? s * P.hz : __builtin_fminf(__builtin_fmaxf(ph.p.z, -P.hz), P.hz);
+    ++ph.nb;
+  }
+  if (scatter) ++ph.ns;
+
+  // cos_i = dot(d, outward normal) is exactly +-d[axis] (the zero components
+  // contribute signed zeros that vanish in the sum).
+  const float d_axis = axis == 0 ? ph.d.x : (axis == 1 ? ph.d.y : ph.d.z);
+  const float cos_i = plus ? d_axis : -d_axis;
+
+  const bool readout = boundary && face == kReadoutFace;
+  const bool side = boundary && face != kReadoutFace;
+  const bool polished = side && P.surface == kPolished;
+  const bool black = side && P.surface == kBlack;
+  const bool refl = side && P.surface >= kSpecular;
+  const bool lambert = P.surface == kLambertian;
+
+  // --- Fresnel (fresnel.hpp:30-46), once for readout and polished lanes -----
+  float prob = P.reflectivity;
+  if (readout || polished) {
+    const float n1 = P.n_scint;
+    const float n2 = readout ? P.n_det : P.n_out;
+    const float eta2 = readout ? P.eta2_det : P.eta2_out;
+    const bool matched = readout ? P.det_matched : P.out_matched;
+    const float sin2_t = eta2 * __builtin_fmaxf(0.0f, 1.0f - cos_i * cos_i);
+    const float cos_t = f_sqrt(1.0f - sin2_t);
+    const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
+    const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
+    const float R = 0.5f * (rs * rs + rp * rp);
+    prob = matched ? 0.0f : (sin2_t >= 1.0f ? 1.0f : R);
+  }
+  // --- the surface decision draw (u <= prob survives, transport.hpp:41) -----
+  const bool decide = readout || polished || refl;
+  const bool survived = decide && to_unit(rng.peek<2>()) <= prob;
+  const bool lambert_reflect = refl && lambert && survived;
+
+  // --- new direction ----------------------------------------------------------
+  if (survived && !lambert_reflect) {  // specular: flip the approached component
+    if (axis == 0) ph.d.x = -ph.d.x;
+    if (axis == 1) ph.d.y = -ph.d.y;
+    if (axis == 2) ph.d.z = -ph.d.z;
+  }
+  if (scatter || lambert_reflect) {
+    // Rayleigh: about the direction, cos from Cardano, draws 3, 4.
+    // Lambertian: about the inward normal -n, cos = sqrt(u), draws 4, 5.
+    const uint32_t wc = scatter ? rng.peek<2>() : rng.peek<3>();
+    const uint32_t wp = scatter ? rng.peek<3>() : rng.peek<4>();
+    const float uc = to_unit(wc), up = to_unit(wp);
+    float cos_theta;
+    if (scatter)
+      cos_theta = rayleigh_cos(uc);
+    else
+      cos_theta = f_sqrt(uc);
+    const float s = plus ? 1.0f : -1.0f;
+    // -n keeps the reference's signed zeros: -(+0) = -0 off-axis.
+    const V3 inward{axis == 0 ? -s : -0.0f, axis == 1 ? -s : -0.0f, axis == 2 ? -s : -0.0f};
+    ph.d = rotate_about(scatter ? ph.d : inward, cos_theta, up);
+  }
+  rng.pos += 2u + (decide ? 1u : 0u) + ((scatter || lambert_reflect) ? 2u : 0u);
+
+  // --- fate -----------------------------------------------------------------
+  const bool alive = scatter || survived;
+  if (!alive) {
+    fate = absorb ? kAbsorbedBulk
+                  : (readout ? kDetected : (polished ? kEscaped : kAbsorbedSurface));
+  }
+  info.absorb = absorb;
+  info.scatter = scatter;
+  info.readout = readout;
+  info.polished = polished;
+  info.black = black;
+  info.refl = refl;
+  info.survived = survived;
+  info.lambert_reflect = lambert_reflect;
+  return alive;
+}
+
+// Tally binning (transport.hpp:170-181); `time_scale` = time_bins / time_max
+// and `two_hx`, `two_hy` = 2 hx, 2 hy, computed once on the host.
+__device__ __forceinline__ uint32_t time_bin(float t, float time_scale, uint32_t bins) {
+  const float f = t * time_scale;
+  return (f >= static_cast<float>(bins)) ? bins : static_cast<uint32_t>(f);
+}
+
+__device__ __forceinline__ uint32_t xy_bin(float x, float y, float hx, float hy, float two_hx,
+                                           float two_hy, uint32_t nb_bins) {
+  const float nb = static_cast<float>(nb_bins);
+  const uint32_t ix = static_cast<uint32_t>(__builtin_fminf((x + hx) / two_hx * nb, nb - 1.0f));
+  const uint32_t iy = static_cast<uint32_t>(__builtin_fminf((y + hy) / two_hy * nb, nb - 1.0f));
+  return iy * nb_bins + ix;
+}
+
+}  // namespace ophip

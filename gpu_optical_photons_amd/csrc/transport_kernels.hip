@@ -1,0 +1,386 @@
+// transport_kernels.hip -- optical-photon transport kernels for gfx950.
+//
+// Two drivers of the same per-lane physics (photon.hpp):
+//
+//  * history_kernel: one lane per photon for the photon's whole life (the
+//    reference's history-based kernels, gpu_backend.cu:47-100). A wave runs
+//    as long as its longest history.
+//
+//  * wavefront_kernel: a persistent grid. Every lane holds one photon in
+//    registers; each wave claims photon ids from a device work queue in
+//    chunks (one u64 atomic per chunk) and, every `steps_per_refill` steps,
+//    re-fills the lanes whose photon has finished. The refill is a wave64
+//    stream compaction of the work queue onto the dead lanes: ballot of the
+//    dead lanes, popcount for the claim, mbcnt for each lane's rank. Photon
+//    state never leaves the registers, so the wave stays full until the queue
+//    drains and no state traffic goes to HBM. This replaces the reference's
+//    relaunch-and-compact loop (gpu_event.cu:125-170) and its per-launch host
+//    round trip; results are identical because a photon's history depends
+//    only on (seed, photon id).
+//
+// Loops whose trip count differs between lanes are written as wave-uniform
+// loops driven by ballots, so the per-wave counters stay in SGPRs and no
+// value is observed at a lane-dependent loop exit.
+//
+// Tallies: each block accumulates u32 histograms [fates | time | xy] in LDS,
+// writes them once to its row of a [grid][words] buffer, and a reduction
+// kernel sums the rows into the u64 tally. Integer sums: the result does not
+// depend on block size, grid size or scheduling.
+#include "kernels.hpp"
+
+namespace ophip {
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Rank of this lane among the set bits of `mask` below it.
+__device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ uint64_t first_lane_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
+
+// Wave-uniform branch-event counters (kept in SGPRs).
+struct WaveCounters {
+  uint64_t v[kCntSlots];
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int i = 0; i < kCntSlots; ++i) v[i] = 0;
+  }
+  __device__ __forceinline__ void add_step(uint64_t active, const StepInfo& s) {
+    v[kCntUseful] += popc(active);
+    v[kCntIssued] += 64;
+    v[kCntAbsorb] += popc(__ballot(s.absorb));
+    v[kCntScatter] += popc(__ballot(s.scatter));
+    v[kCntReadout] += popc(__ballot(s.readout));
+    v[kCntPolished] += popc(__ballot(s.polished));
+    v[kCntBlack] += popc(__ballot(s.black));
+    v[kCntReflector] += popc(__ballot(s.refl));
+    v[kCntLambert] += popc(__ballot(s.lambert_reflect));
+  }
+  __device__ __forceinline__ void flush(unsigned long long* dst) const {
+    if (lane_id() == 0) {
+#pragma unroll
+      for (int i = 0; i < kCntSlots; ++i)
+        if (v[i]) atomicAdd(dst + i, static_cast<unsigned long long>(v[i]));
+    }
+  }
+};
+
+__device__ __forceinline__ void zero_lds(uint32_t* lds, uint32_t words) {
+  for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) lds[w] = 0u;
+}
+
+// Output of one finished photon: its record at `idx` and/or its tally entries.
+template <bool kRec, bool kTally>
+__device__ __forceinline__ void finish(const DevParams& P, const LaunchOut& out, uint32_t* lds,
+                                       uint64_t idx, const Photon& ph, uint32_t fate,
+                                       uint64_t& snb, uint64_t& sns) {
+  if (kRec) {
+    out.rec.fate[idx] = static_cast<uint8_t>(fate);
+    out.rec.nb[idx] = ph.nb;
+    out.rec.ns[idx] = ph.ns;
+    out.rec.t[idx] = ph.t;
+    out.rec.x[idx] = ph.p.x;
+    out.rec.y[idx] = ph.p.y;
+    out.rec.z[idx] = ph.p.z;
+    out.rec.path[idx] = ph.path;
+  }
+  if (kTally) {
+    const DevTally& T = out.tally;
+    atomicAdd(&lds[fate], 1u);
+    if (fate == kDetected) {
+      atomicAdd(&lds[5u + time_bin(ph.t, T.time_scale, T.time_bins)], 1u);
+      atomicAdd(&lds[6u + T.time_bins + xy_bin(ph.p.x, ph.p.y, P.hx, P.hy, T.two_hx, T.two_hy,
+                                               T.xy_bins)],
+                1u);
+    }
+    snb += ph.nb;
+    sns += ph.ns;
+  }
+}
+
+template <bool kTally>
+__device__ __forceinline__ void flush_block(const LaunchOut& out, uint32_t* lds, uint64_t snb,
+                                            uint64_t sns, const WaveCounters& wc) {
+  wc.flush(out.counters);
+  if (kTally) {
+    const uint64_t bnb = wave_sum_u64(snb), bns = wave_sum_u64(sns);
+    if (lane_id() == 0) {
+      if (bnb) atomicAdd(out.sums + 2u * blockIdx.x, static_cast<unsigned long long>(bnb));
+      if (bns) atomicAdd(out.sums + 2u * blockIdx.x + 1u, static_cast<unsigned long long>(bns));
+    }
+    __syncthreads();
+    uint32_t* dst = out.partials + static_cast<size_t>(blockIdx.x) * out.tally.words;
+    for (uint32_t w = threadIdx.x; w < out.tally.words; w += blockDim.x) dst[w] = lds[w];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// History-based kernel. Records: the grid covers n, lane i <-> photon i.
+// Tally: waves stride over the photons in steps of the whole grid.
+template <bool kRec, bool kTally>
+__global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey key,
+                                                       uint64_t first_id, uint64_t n,
+                                                       LaunchOut out) {
+  extern __shared__ uint32_t lds[];
+  if (kTally) {
+    zero_lds(lds, out.tally.words);
+    __syncthreads();
+  }
+  const uint32_t lane = lane_id();
+  const uint64_t wave0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x - lane);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  WaveCounters wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;
+
+  for (uint64_t base = wave0; base < n; base += stride) {  // uniform per wave
+    const uint64_t i = base + lane;
+    bool alive = i < n;
+    Stream rng;
+    Photon ph;
+    uint32_t fate = kMaxSteps;
+    if (alive) {
+      rng.start(first_id + i, key, 0u);
+      emit(P, rng, ph);
+    }
+    wc.v[kCntEmitted] += popc(__ballot(alive));
+    const bool valid = alive;
+    alive = alive && P.max_steps > 0u;
+    uint32_t done = 0;
+    for (;;) {
+      const uint64_t act = __ballot(alive);
+      if (act == 0) break;
+      StepInfo info{};
+      if (alive) {
+        rng.refill(key);
+        const bool still = step(P, rng, ph, fate, info);
+        ++done;
+        alive = still && done < P.max_steps;
+        if (still && !alive) fate = kMaxSteps;
+      }
+      wc.add_step(act, info);
+    }
+    wc.v[kCntDetected] += popc(__ballot(valid && fate == kDetected));
+    if (valid) finish<kRec, kTally>(P, out, lds, i, ph, fate, snb, sns);
+  }
+  if (kTally) __syncthreads();
+  flush_block<kTally>(out, lds, snb, sns, wc);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent wavefront kernel (see the file comment).
+template <bool kRec, bool kTally>
+__global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P, PhiloxKey key,
+                                                         uint64_t first_id, uint64_t n,
+                                                         uint32_t steps_per_refill, uint32_t chunk,
+                                                         unsigned long long* queue,
+                                                         LaunchOut out) {
+  extern __shared__ uint32_t lds[];
+  if (kTally) {
+    zero_lds(lds, out.tally.words);
+    __syncthreads();
+  }
+  const uint32_t lane = lane_id();
+  WaveCounters wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;
+
+  uint64_t q_next = 0, q_end = 0;  // photons claimed by this wave, not yet started
+  bool q_done = false;             // the global queue is exhausted
+  bool alive = false;
+  uint64_t idx = 0;
+  uint32_t done = 0, fate = kMaxSteps;
+  Stream rng;
+  Photon ph;
+
+  for (;;) {
+    // ---- refill: compact queued photons onto the dead lanes ----------------
+    const uint64_t dead = __ballot(!alive);
+    if (dead != 0 && !q_done) {
+      const uint32_t need = popc(dead);
+      const uint64_t have = q_end - q_next;
+      const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+      const uint64_t b1 = q_next;
+      q_next += t1;
+      uint64_t b2 = 0;
+      uint32_t t2 = 0;
+      if (t1 < need) {
+        unsigned long long got = 0;
+        if (lane == 0) got = atomicAdd(queue, static_cast<unsigned long long>(chunk));
+        got = first_lane_u64(got);
+        if (got >= n) {
+          q_done = true;
+        } else {
+          q_end = (got + chunk < n) ? got + chunk : n;
+          const uint64_t avail = q_end - got;
+          t2 = (need - t1 < avail) ? need - t1 : static_cast<uint32_t>(avail);
+          b2 = got;
+          q_next = got + t2;
+        }
+      }
+      const uint32_t r = rank_in(dead);
+      bool fresh = false;
+      if (!alive && r < t1 + t2) {
+        idx = r < t1 ? b1 + r : b2 + (r - t1);
+        rng.start(first_id + idx, key, 0u);
+        emit(P, rng, ph);
+        done = 0;
+        fate = kMaxSteps;
+        alive = P.max_steps > 0u;
+        fresh = true;
+      }
+      wc.v[kCntEmitted] += t1 + t2;
+      if (fresh && !alive) {  // max_steps == 0: finished at emission
+        finish<kRec, kTally>(P, out, lds, idx, ph, fate, snb, sns);
+      }
+    }
+    if (__ballot(alive) == 0) {
+      if (q_done) break;
+      continue;
+    }
+
+    // ---- up to steps_per_refill transport steps -----------------------------
+    for (uint32_t k = 0; k < steps_per_refill; ++k) {
+      const uint64_t act = __ballot(alive);
+      if (act == 0) break;
+      StepInfo info{};
+      bool finished = false;
+      if (alive) {
+        rng.refill(key);
+        const bool still = step(P, rng, ph, fate, info);
+        ++done;
+        if (!still || done >= P.max_steps) {
+          if (still) fate = kMaxSteps;
+          finished = true;
+          alive = false;
+        }
+      }
+      wc.add_step(act, info);
+      wc.v[kCntDetected] += popc(__ballot(finished && fate == kDetected));
+      if (finished) finish<kRec, kTally>(P, out, lds, idx, ph, fate, snb, sns);
+    }
+  }
+  if (kTally) __syncthreads();
+  flush_block<kTally>(out, lds, snb, sns, wc);
+}
+
+__global__ void tally_reduce_kernel(uint32_t words, int blocks, const uint32_t* partials,
+                                    const unsigned long long* sums, unsigned long long* tally) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < words) {
+    unsigned long long s = 0;
+    for (int b = 0; b < blocks; ++b) s += partials[static_cast<size_t>(b) * words + w];
+    tally[w] = s;
+  } else if (w < words + 2u) {
+    const uint32_t j = w - words;
+    unsigned long long s = 0;
+    for (int b = 0; b < blocks; ++b) s += sums[2 * b + j];
+    tally[w] = s;
+  }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, uint64_t n) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    p[i] = 0ull;
+}
+
+template <bool R, bool T>
+void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n, int block,
+                      int grid, const LaunchOut& out, hipStream_t stream) {
+  const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
+  hipLaunchKernelGGL((history_kernel<R, T>), dim3(grid), dim3(block), lds, stream, P, key,
+                     first_id, n, out);
+}
+
+template <bool R, bool T>
+void launch_wavefront_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                        const WavefrontShape& s, unsigned long long* queue, const LaunchOut& out,
+                        hipStream_t stream) {
+  const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
+  hipLaunchKernelGGL((wavefront_kernel<R, T>), dim3(s.grid), dim3(s.block), lds, stream, P, key,
+                     first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+}
+
+template <class K>
+int blocks_per_cu(K kernel, int block, size_t lds) {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, block, lds) != hipSuccess) return 0;
+  return b;
+}
+
+}  // namespace
+
+void launch_history(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n, int block,
+                    int grid, const LaunchOut& out, hipStream_t stream) {
+  if (out.records && out.tally_on)
+    launch_history_t<true, true>(P, key, first_id, n, block, grid, out, stream);
+  else if (out.records)
+    launch_history_t<true, false>(P, key, first_id, n, block, grid, out, stream);
+  else if (out.tally_on)
+    launch_history_t<false, true>(P, key, first_id, n, block, grid, out, stream);
+  else
+    launch_history_t<false, false>(P, key, first_id, n, block, grid, out, stream);
+}
+
+void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                      const WavefrontShape& shape, unsigned long long* queue, const LaunchOut& out,
+                      hipStream_t stream) {
+  if (out.records && out.tally_on)
+    launch_wavefront_t<true, true>(P, key, first_id, n, shape, queue, out, stream);
+  else if (out.records)
+    launch_wavefront_t<true, false>(P, key, first_id, n, shape, queue, out, stream);
+  else if (out.tally_on)
+    launch_wavefront_t<false, true>(P, key, first_id, n, shape, queue, out, stream);
+  else
+    launch_wavefront_t<false, false>(P, key, first_id, n, shape, queue, out, stream);
+}
+
+void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials,
+                         const unsigned long long* sums, unsigned long long* tally,
+                         hipStream_t stream) {
+  const uint32_t total = t.words + 2u;
+  const int threads = 256;
+  hipLaunchKernelGGL(tally_reduce_kernel, dim3((total + threads - 1) / threads), dim3(threads), 0,
+                     stream, t.words, blocks, partials, sums, tally);
+}
+
+void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {
+  if (n == 0) return;
+  const int threads = 256;
+  const uint64_t want = (n + threads - 1) / threads;
+  const int grid = static_cast<int>(want < 1024 ? want : 1024);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid), dim3(threads), 0, stream, p, n);
+}
+
+int history_blocks_per_cu(bool tally, int block, size_t lds) {
+  return tally ? blocks_per_cu(history_kernel<false, true>, block, lds)
+               : blocks_per_cu(history_kernel<true, false>, block, lds);
+}
+
+int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds) {
+  if (records && tally) return blocks_per_cu(wavefront_kernel<true, true>, block, lds);
+  if (records) return blocks_per_cu(wavefront_kernel<true, false>, block, lds);
+  if (tally) return blocks_per_cu(wavefront_kernel<false, true>, block, lds);
+  return blocks_per_cu(wavefront_kernel<false, false>, block, lds);
+}
+
+}  // namespace ophip

@@ -1,0 +1,177 @@
+/*
+ * optphot_hip.h -- C-ABI of the MI355X (gfx950) optical-photon transport
+ * library, liboptphot_hip.so.
+ *
+ * Plain C: POD structs, plain pointers and sizes. Every entry point is
+ * synchronous (it returns when the results are in the caller's memory),
+ * returns OPTPHOT_OK or an error code, and leaves a message for
+ * optphot_last_error(). Calls are serialised per process by the library.
+ *
+ * The first group replaces the reference's GPU backend entry points
+ * (include/optphot/backend.hpp:84-115); the C++ header optphot_hip/backend.hpp
+ * re-exposes them with the reference's exact C++ signatures and exception
+ * types. The second group is new: the persistent wavefront tally, batched
+ * events and the adjoint.
+ */
+#ifndef OPTPHOT_HIP_H
+#define OPTPHOT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. INVALID_ARGUMENT corresponds to the reference's
+ * std::invalid_argument (gpu_event.cu:177-182), RUNTIME to std::runtime_error
+ * (cuda_util.cuh:17-24, gpu_backend.cu:171-185). */
+#define OPTPHOT_OK 0
+#define OPTPHOT_ERR_INVALID_ARGUMENT 1
+#define OPTPHOT_ERR_RUNTIME 2
+#define OPTPHOT_ERR_NO_DEVICE 3
+
+/* Surface / source / fate codes: include/optphot/params.hpp:15-35. */
+enum { OPTPHOT_SURFACE_POLISHED = 0, OPTPHOT_SURFACE_BLACK = 1, OPTPHOT_SURFACE_SPECULAR = 2,
+       OPTPHOT_SURFACE_LAMBERTIAN = 3 };
+enum { OPTPHOT_SOURCE_POINT = 0, OPTPHOT_SOURCE_VOLUME = 1 };
+enum { OPTPHOT_FATE_DETECTED = 0, OPTPHOT_FATE_ABSORBED_BULK = 1,
+       OPTPHOT_FATE_ABSORBED_SURFACE = 2, OPTPHOT_FATE_ESCAPED = 3, OPTPHOT_FATE_MAX_STEPS = 4 };
+
+/* SimParams, include/optphot/params.hpp:40-63: 72 bytes, same field order,
+ * lengths in mm, times in ns, +inf disables a bulk process. */
+typedef struct optphot_sim_params {
+  float hx, hy, hz;
+  float n_scint, n_det, n_out;
+  float abs_length, scat_length;
+  uint32_t surface;
+  float reflectivity;
+  uint32_t source;
+  float src_x, src_y, src_z;
+  float tau;
+  uint32_t max_steps;
+  uint64_t seed;
+} optphot_sim_params;
+
+/* TallyConfig, include/optphot/transport.hpp:164-168. */
+typedef struct optphot_tally_config {
+  uint32_t time_bins;
+  float time_max;
+  uint32_t xy_bins;
+} optphot_tally_config;
+
+/* RunTiming, include/optphot/backend.hpp:65-71 (milliseconds). */
+typedef struct optphot_run_timing {
+  double total_ms, kernel_ms, h2d_ms, d2h_ms, alloc_ms;
+} optphot_run_timing;
+
+/* EventStats, include/optphot/backend.hpp:95-103, with wave64 lanes:
+ * issued_lane_steps = sum over waves and steps of 64 x (any lane stepping). */
+typedef struct optphot_event_stats {
+  uint64_t launches, useful_lane_steps, issued_lane_steps;
+  double kernels_ms;
+} optphot_event_stats;
+
+/* Caller-owned host arrays of n elements each: PhotonRecords' structure of
+ * arrays (backend.hpp:31-50, 29 bytes per photon), indexed by photon number. */
+typedef struct optphot_records {
+  uint8_t* fate;
+  uint32_t* n_boundary;
+  uint32_t* n_scatter;
+  float *t, *x, *y, *z, *path;
+} optphot_records;
+
+/* Flat tally layout (u64 words), the reference's device layout
+ * (gpu_backend.cu:56-66): [fate counts 5 | time hist time_bins+1 |
+ * xy hist xy_bins^2 row-major [iy][ix] | boundary_sum | scatter_sum]. */
+#define OPTPHOT_TALLY_WORDS(c) (5u + (c).time_bins + 1u + (c).xy_bins * (c).xy_bins + 2u)
+
+/* Branch-event counters of a run, for the algorithmic-work accounting. */
+enum {
+  OPTPHOT_CNT_USEFUL_LANE_STEPS = 0, /* photon-steps performed */
+  OPTPHOT_CNT_ISSUED_LANE_STEPS = 1, /* 64 x wave-steps with any lane active */
+  OPTPHOT_CNT_ABSORB = 2,
+  OPTPHOT_CNT_SCATTER = 3,
+  OPTPHOT_CNT_READOUT = 4,       /* readout-face hits (Fresnel to n_det) */
+  OPTPHOT_CNT_POLISHED = 5,      /* polished side hits (Fresnel to n_out) */
+  OPTPHOT_CNT_BLACK = 6,         /* black side hits */
+  OPTPHOT_CNT_REFLECTOR = 7,     /* specular / Lambertian side hits */
+  OPTPHOT_CNT_LAMBERT_REFLECT = 8,
+  OPTPHOT_CNT_EMITTED = 9,
+  OPTPHOT_CNT_DETECTED = 10,
+  OPTPHOT_CNT_SLOTS = 16
+};
+typedef struct optphot_step_counters {
+  uint64_t v[OPTPHOT_CNT_SLOTS];
+} optphot_step_counters;
+
+/* Launch shape of the persistent wavefront kernel. Zero fields are filled
+ * from the autotuner's choice for the run (or built-in defaults). */
+typedef struct optphot_wavefront_tuning {
+  int block_size;             /* threads per block, multiple of 64, <= 1024 */
+  uint32_t steps_per_refill;  /* transport steps between work-queue refills */
+  uint32_t queue_chunk;       /* photons a wave claims per queue atomic */
+  int blocks_per_cu;          /* persistent blocks per compute unit */
+} optphot_wavefront_tuning;
+
+/* ---- Reference GPU backend entry points ----------------------------------- */
+
+/* gpu_available(), backend.hpp:84 */
+int optphot_gpu_available(void);
+/* gpu_device_name(), backend.hpp:85; writes a NUL-terminated name. */
+int optphot_gpu_device_name(char* buf, size_t len);
+/* Message of the last failed call on this thread ("" if none). */
+const char* optphot_last_error(void);
+
+/* run_gpu_records(p, n, out, block_size=256, first_id=0, chunk=1<<24),
+ * backend.hpp:88-89 / gpu_backend.cu:119-154: history-based kernel, one lane
+ * per photon, records of photons first_id .. first_id+n-1 into `out`.
+ * block_size: multiple of 64 in [64, 1024]. `timing` may be NULL. */
+int optphot_run_gpu_records(const optphot_sim_params* p, uint64_t n, const optphot_records* out,
+                            int block_size, uint64_t first_id, uint64_t chunk,
+                            optphot_run_timing* timing);
+
+/* run_gpu_tally(p, c, n, out, block_size=256, first_id=0), backend.hpp:90-91 /
+ * gpu_backend.cu:158-212: history-based tally into `tally`
+ * (OPTPHOT_TALLY_WORDS(*c) u64 words). */
+int optphot_run_gpu_tally(const optphot_sim_params* p, const optphot_tally_config* c, uint64_t n,
+                          uint64_t* tally, int block_size, uint64_t first_id,
+                          optphot_run_timing* timing);
+
+/* run_gpu_records_event(p, n, out, steps_per_launch, block_size, first_id,
+ * chunk, stats), backend.hpp:112-115 / gpu_event.cu:174-244: the same records
+ * as optphot_run_gpu_records, bit for bit, computed by the persistent
+ * wavefront kernel with `steps_per_launch` steps between refills. One launch
+ * per chunk. `stats` and `timing` may be NULL. */
+int optphot_run_gpu_records_event(const optphot_sim_params* p, uint64_t n,
+                                  const optphot_records* out, uint32_t steps_per_launch,
+                                  int block_size, uint64_t first_id, uint64_t chunk,
+                                  optphot_event_stats* stats, optphot_run_timing* timing);
+
+/* ---- New entry points (no counterpart in the reference) -------------------- */
+
+/* Tally of photons first_id .. first_id+n-1 by the persistent wavefront kernel
+ * (device work queue, refill of finished lanes, LDS-privatised histograms).
+ * Same tally as optphot_run_gpu_tally, word for word. tuning / stats /
+ * counters / timing may be NULL. */
+int optphot_run_gpu_tally_wavefront(const optphot_sim_params* p, const optphot_tally_config* c,
+                                    uint64_t n, uint64_t* tally, uint64_t first_id,
+                                    const optphot_wavefront_tuning* tuning,
+                                    optphot_event_stats* stats, optphot_step_counters* counters,
+                                    optphot_run_timing* timing);
+
+/* Branch-event counters of the history-based tally kernel (same work as
+ * optphot_run_gpu_tally). */
+int optphot_run_gpu_tally_counters(const optphot_sim_params* p, const optphot_tally_config* c,
+                                   uint64_t n, uint64_t* tally, int block_size, uint64_t first_id,
+                                   optphot_step_counters* counters, optphot_run_timing* timing);
+
+/* Number of compute units and the kernels' register / occupancy facts,
+ * written as "key=value" lines into buf (diagnostics). */
+int optphot_device_info(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPTPHOT_HIP_H */

@@ -1,0 +1,105 @@
+"""GPU parity: the HIP kernels against the oracle, photon by photon.
+
+The oracle is the plain-C restatement of the reference's CPU path
+(oracle/optphot_oracle.c) in the reference's bit-reproducible math mode
+(OPTPHOT_PORTABLE_MATH); the kernels compute in the same mode, so every
+record must be identical to the last bit (the reference's own bar for its
+reproducible build, tests/test_gpu.cpp:91-94).
+"""
+import numpy as np
+import pytest
+
+from tests import configs
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path")
+
+
+def assert_records_identical(gpu, ref, label=""):
+    for f in FIELDS:
+        a = getattr(gpu, f) if not isinstance(gpu, dict) else gpu[f]
+        b = ref[f] if isinstance(ref, dict) else getattr(ref, f)
+        if not np.array_equal(a.view(np.uint8), b.view(np.uint8)):
+            bad = np.nonzero(a.view(f"u{a.itemsize}") != b.view(f"u{b.itemsize}"))[0]
+            i = int(bad[0])
+            raise AssertionError(f"{label}: field {f} differs at {len(bad)} photons; first id {i}: "
+                                 f"gpu {a[i]!r} oracle {b[i]!r}")
+
+
+@pytest.mark.parametrize("name", sorted(configs.PARITY))
+def test_history_records_bitwise(backend, oracle_portable, name):
+    p = configs.PARITY[name]
+    n = 200_003
+    print(f"[parity] history records {name} n={n}", flush=True)
+    gpu, tm = backend.run_gpu_records(p, n)
+    ref = oracle_portable.records(p, n)
+    assert_records_identical(gpu, ref, name)
+
+
+def test_records_first_id_and_chunk(backend, oracle_portable):
+    p = configs.every_process()
+    n = 300_000
+    a, _ = backend.run_gpu_records(p, n, block_size=256, first_id=0, chunk=1 << 24)
+    b, _ = backend.run_gpu_records(p, n, block_size=64, first_id=0, chunk=70001)
+    assert_records_identical(a, b.as_dict(), "chunk/block")
+    tail, _ = backend.run_gpu_records(p, 500, first_id=1000)
+    ref = oracle_portable.records(p, 500, first_id=1000)
+    assert_records_identical(tail, ref, "first_id")
+    assert np.array_equal(tail.t, a.t[1000:1500])
+
+
+@pytest.mark.parametrize("name", ["default", "wrapped_lambertian", "capped", "c2_long"])
+def test_event_records_equal_history_for_any_k(backend, name):
+    p = configs.PARITY[name]
+    n = 100_003
+    hist, _ = backend.run_gpu_records(p, n)
+    steps = int(hist.steps().sum())
+    for k in (1, 3, 100_000):
+        print(f"[parity] event {name} k={k}", flush=True)
+        ev, tm, st = backend.run_gpu_records_event(p, n, k, block_size=64, chunk=30011)
+        assert_records_identical(ev, hist.as_dict(), f"{name} k={k}")
+        assert st.useful_lane_steps == steps
+        assert st.issued_lane_steps >= st.useful_lane_steps
+        assert st.launches == (n + 30010) // 30011
+
+
+def test_single_launch_issued_lane_steps(backend):
+    """With steps_per_launch >= max_steps each wave runs 64 consecutive photons
+    to completion: issued = sum over 64-photon groups of 64 x longest."""
+    p = configs.default()
+    n = 200_000
+    hist, _ = backend.run_gpu_records(p, n)
+    ev, tm, st = backend.run_gpu_records_event(p, n, p.max_steps, block_size=256, chunk=n)
+    steps = hist.steps()
+    pad = (-n) % 64
+    longest = np.concatenate([steps, np.zeros(pad, steps.dtype)]).reshape(-1, 64).max(axis=1)
+    assert st.launches == 1
+    assert st.issued_lane_steps == int(64 * longest.sum())
+
+
+@pytest.mark.parametrize("name", ["default", "c1_plain", "c2_long", "wrapped_lambertian",
+                                  "absorption"])
+def test_tally_equals_oracle(backend, oracle_portable, name):
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = configs.PARITY[name]
+    n = 1_000_000
+    c = TallyConfig(256, 100.0, 64) if name == "c2_long" else TallyConfig()
+    ref = oracle_portable.tally(p, n, c.time_bins, c.time_max, c.xy_bins)
+    for block in (256, 128):
+        t, tm = backend.run_gpu_tally(p, c, n, block_size=block)
+        np.testing.assert_array_equal(t.fate_counts, ref["fate_counts"])
+        np.testing.assert_array_equal(t.time_hist, ref["time_hist"])
+        np.testing.assert_array_equal(t.xy_hist, ref["xy_hist"])
+        assert t.boundary_sum == ref["boundary_sum"]
+        assert t.scatter_sum == ref["scatter_sum"]
+    for tuning in (None, dict(block_size=64, steps_per_refill=1, queue_chunk=64),
+                   dict(block_size=1024, steps_per_refill=16, queue_chunk=4096)):
+        t, tm, st, cnt = backend.run_gpu_tally_wavefront(p, c, n, tuning=tuning)
+        np.testing.assert_array_equal(t.flat(), backend.Tally.from_flat(
+            np.concatenate([ref["fate_counts"], ref["time_hist"], ref["xy_hist"].ravel(),
+                            np.array([ref["boundary_sum"], ref["scatter_sum"]], np.uint64)]),
+            c).flat())
+        steps = ref["boundary_sum"] + ref["scatter_sum"] + int(ref["fate_counts"][1])
+        assert cnt.as_dict()["useful_lane_steps"] == steps
+        assert cnt.as_dict()["emitted"] == n
