@@ -1,0 +1,111 @@
+"""Autotuner for the launch shape of the persistent wavefront kernels.
+
+Tunes block size, steps between refills ("wavefront length"), the queue
+chunk a wave claims per atomic, and persistent blocks per CU, by timing the
+real kernel on a sample of the workload. Results never depend on the choice
+(integer tallies, per-photon streams; checked by the sweep itself and by the
+GPU tests), only the speed does. Choices persist in tuning.json next to this
+file, keyed by a workload signature (kernel, surface model, source, which bulk
+processes are on, tally size class).
+
+    python -m gpu_optical_photons_amd.autotune   # sweep the standard workloads
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import math
+import os
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuning.json")
+
+DEFAULT = {"block_size": 256, "steps_per_refill": 4, "queue_chunk": 256, "blocks_per_cu": 0}
+
+BLOCKS = (64, 128, 256, 512, 1024)
+STEPS = (1, 2, 4, 8, 16)
+CHUNKS = (128, 512, 2048)
+
+
+def signature(kernel: str, p, c=None) -> str:
+    words = 0 if c is None else c.words
+    size = "small" if words < 1024 else ("mid" if words < 8192 else "large")
+    return (f"{kernel}|surface={int(p.surface)}|source={int(p.source)}"
+            f"|abs={'on' if not math.isinf(p.abs_length) else 'off'}"
+            f"|scat={'on' if not math.isinf(p.scat_length) else 'off'}|tally={size}")
+
+
+def _load():
+    if os.path.exists(CACHE):
+        try:
+            return json.load(open(CACHE))
+        except Exception:
+            return {}
+    return {}
+
+
+def choice_for(kernel: str, p, c=None) -> dict:
+    entry = _load().get(signature(kernel, p, c))
+    if entry:
+        return dict(entry["tuning"])
+    return dict(DEFAULT)
+
+
+def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS, reps: int = 2,
+          save: bool = True, log=print) -> dict:
+    """Times every candidate on n photons; keeps the fastest. Every candidate
+    must give the same tally word for word."""
+    import numpy as np
+
+    from . import backend
+
+    if kernel != "wavefront_tally":
+        raise ValueError(f"unknown kernel {kernel!r}")
+    ref = None
+    results = []
+    for b, k, ch in itertools.product(blocks, steps, chunks):
+        tu = {"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0}
+        best = math.inf
+        for _ in range(reps):
+            t, tm, st, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tu)
+            best = min(best, tm.kernel_ms)
+        flat = t.flat()
+        if ref is None:
+            ref = flat
+        elif not np.array_equal(ref, flat):
+            raise AssertionError(f"tally differs for tuning {tu}: the kernel is not "
+                                 "schedule-independent")
+        results.append((best, tu))
+        log(f"  {kernel} block={b:4d} steps={k:2d} chunk={ch:5d}: {best:8.3f} ms")
+    results.sort(key=lambda r: r[0])
+    best_ms, best_tu = results[0]
+    entry = {"tuning": best_tu, "kernel_ms": best_ms, "photons": n,
+             "photons_per_s": n / (best_ms * 1e-3),
+             "default_ms": next((ms for ms, tu in results if tu == DEFAULT), None),
+             "swept": len(results), "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+    if save:
+        cache = _load()
+        cache[signature(kernel, p, c)] = entry
+        with open(CACHE, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return entry
+
+
+def main():
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from .params import SimParams, Source, Surface, TallyConfig
+
+    c2 = SimParams(hx=50.0, hy=50.0, hz=25.0, n_det=1.5, abs_length=2000.0, scat_length=300.0,
+                   surface=Surface.LAMBERTIAN, reflectivity=0.97, source=Source.VOLUME, seed=1)
+    c1 = SimParams(n_det=1.5, abs_length=1000.0, seed=1)
+    for name, p, c in (("long histories", c2, TallyConfig(256, 100.0, 64)),
+                       ("plain small", c1, TallyConfig())):
+        print(f"sweep: {name}", flush=True)
+        e = sweep("wavefront_tally", p, c, 20_000_000, log=lambda s: print(s, flush=True))
+        print(f"best: {e['tuning']} {e['kernel_ms']:.3f} ms", flush=True)
+
+
+if __name__ == "__main__":
+    main()

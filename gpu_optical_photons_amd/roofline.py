@@ -1,0 +1,93 @@
+"""Algorithmic work of the transport, for roofline fractions.
+
+The count is of the reference algorithm's operations, not of the
+instructions a kernel happens to issue. Rule: every IEEE f32 operation
+(+ - * /, sqrt, min/max, compare, select, negation, int<->float conversion)
+and every 32-bit integer operation counts 1; a 32x32->64 multiply counts 2
+(low and high halves). Transcendentals count as the operations of the
+reference's portable implementations. The per-branch costs below were counted
+by hand from the reference's formulas and are frozen here:
+
+  Philox4x32-10 block      philox.hpp:54-76   10 x (4 mul halves + 4 xor) + 18 key adds = 98
+  uniform draw             philox.hpp:82-84   98 / 4 + 4 (shift, add, cvt, mul)        = 28.5
+  log                      pmath.hpp:62-82                                            29
+  sincos_turns             pmath.hpp:87-119                                           33
+  cbrt                     pmath.hpp:123-130  2 + 4 Newton x 7                         30
+  exponential distance     sampling.hpp:17-19 inf test + neg + mul + log               32 (1 if disabled)
+  box exit                 geometry.hpp:25-46 3 axes x 9 + clamp                       28
+  event decision           transport.hpp:81-90                                         4
+  advance                  transport.hpp:82-84 (pos 6, t 2, path 1, n/c 1)             10
+  boundary common          advance + snap 10 + count 1 + normal 5 + dot 5              31
+  Fresnel                  fresnel.hpp:30-46                                          28
+  reflect                  fresnel.hpp:49-51                                          12
+  rotate_about             vec3.hpp:35-53 (basis 17, sin 4, sincos 33, sum 17, norm 10) 81
+  Rayleigh cos             sampling.hpp:39-44 (q 2, sqrt 3, cbrt 30, select/inv 3, clamp 4) 42
+  emission, point          transport.hpp:55-67 (3 draws, isotropic 42, time 32)         160
+  emission, volume         + 3 draws + sample_in_box 9                                 255
+
+Per step: common = 2 draws + 2 exponential distances + box exit + decision;
+then absorb +10; scatter +10 +2 draws +42 +81 +1; every boundary hit +31;
+readout and polished hits +28 (Fresnel) +28.5 (draw) +1 (test), +12 if
+reflected; black +2; specular / Lambertian +28.5 +1 +2, then +12 (specular
+reflect) or +57 +1 +3 +81 (Lambertian reflect).
+"""
+from __future__ import annotations
+
+import math
+
+# MI355X peaks (published): 256 CUs at 2.4 GHz.
+FP32_VECTOR_PEAK_TFLOPS = 157.3            # vendor FP32 vector figure (packed FMA)
+VALU_ISSUE_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # one lane-op per lane per clock = 39.3
+HBM_PEAK_GBS = 8000.0
+
+DRAW = 98.0 / 4.0 + 4.0
+LOG = 29.0
+EXPD = 3.0 + LOG  # inf test, negation, multiply, log
+BOX = 28.0
+DECIDE = 4.0
+ADVANCE = 10.0
+BOUNDARY = 31.0
+FRESNEL = 28.0
+REFLECT = 12.0
+ROTATE = 81.0
+RAYLEIGH_COS = 42.0
+ISOTROPIC = 42.0
+EMIT_POINT = 3 * DRAW + ISOTROPIC + 1.0 + 2.0 + LOG
+EMIT_VOLUME = EMIT_POINT + 3 * DRAW + 9.0
+
+
+def step_common(abs_length: float, scat_length: float) -> float:
+    e_abs = 1.0 if math.isinf(abs_length) else EXPD
+    e_sca = 1.0 if math.isinf(scat_length) else EXPD
+    return 2 * DRAW + e_abs + e_sca + BOX + DECIDE
+
+
+def algorithmic_ops(params, counters: dict, fate_counts) -> float:
+    """Total operations of the reference algorithm for a run, from the
+    kernel's branch-event counters and the run's fate counts."""
+    c = counters
+    steps = c["useful_lane_steps"]
+    detected = int(fate_counts[0])
+    escaped = int(fate_counts[3])
+    absorbed_surface = int(fate_counts[2])
+    ops = steps * step_common(params.abs_length, params.scat_length)
+    ops += c["absorb"] * ADVANCE
+    ops += c["scatter"] * (ADVANCE + 2 * DRAW + RAYLEIGH_COS + ROTATE + 1.0)
+    hits = c["readout"] + c["polished"] + c["black"] + c["reflector"]
+    ops += hits * BOUNDARY
+    fres = c["readout"] + c["polished"]
+    ops += fres * (FRESNEL + DRAW + 1.0)
+    ops += (c["readout"] - detected + c["polished"] - escaped) * REFLECT
+    ops += c["black"] * 2.0
+    ops += c["reflector"] * (DRAW + 1.0 + 2.0)
+    refl_survived = c["reflector"] - (absorbed_surface - c["black"])
+    lam = c["lambert_reflect"]
+    ops += (refl_survived - lam) * REFLECT
+    ops += lam * (2 * DRAW + 1.0 + 3.0 + ROTATE)
+    emit = EMIT_VOLUME if params.source == 1 else EMIT_POINT
+    ops += c["emitted"] * emit
+    return float(ops)
+
+
+def photon_steps(counters: dict) -> int:
+    return int(counters["useful_lane_steps"])

@@ -127,6 +127,7 @@ class Oracle(_Lib):
                                        _f32p, _f32p, ctypes.c_void_p])
         self._fn("run_tally", None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                      ctypes.c_uint64, ctypes.c_int, _u64p])
+        self._fn("u32_to_uniform", ctypes.c_float, [ctypes.c_uint32])
 
     def records(self, params, n, first_id=0, threads=0, with_draws=False):
         r = _empty_records(n)

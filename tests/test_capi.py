@@ -1,0 +1,65 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point the
+header declares, and rejects invalid arguments before touching the device
+(the reference's std::invalid_argument cases, gpu_event.cu:177-182)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from gpu_optical_photons_amd import backend
+from gpu_optical_photons_amd.params import SimParams, TallyConfig
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"\b(optphot_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_entry_points()
+    assert "optphot_run_gpu_records" in names and len(names) >= 8
+    lib = ctypes.CDLL(backend.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts():
+    assert ctypes.sizeof(SimParams) == 72
+    assert ctypes.sizeof(TallyConfig) == 12
+    assert SimParams.seed.offset == 64 and SimParams.max_steps.offset == 60
+
+
+def test_invalid_block_size_rejected_before_any_gpu_call():
+    p = SimParams()
+    for bs in (0, 32, 96, 2048, -64):
+        with pytest.raises(ValueError, match="block_size"):
+            backend.run_gpu_records(p, 10, block_size=bs)
+        with pytest.raises(ValueError, match="block_size"):
+            backend.run_gpu_records_event(p, 10, 4, block_size=bs)
+        with pytest.raises(ValueError, match="block_size"):
+            backend.run_gpu_tally(p, TallyConfig(), 10, block_size=bs)
+
+
+def test_invalid_steps_and_params_rejected():
+    with pytest.raises(ValueError, match="steps_per_launch"):
+        backend.run_gpu_records_event(SimParams(), 10, 0)
+    with pytest.raises(ValueError, match="surface"):
+        backend.run_gpu_records(SimParams(surface=7), 10)
+    with pytest.raises(ValueError, match="half sizes"):
+        backend.run_gpu_records(SimParams(hx=0.0), 10)
+
+
+def test_no_device_reports_cleanly():
+    """Without a device the library says so instead of crashing."""
+    if backend.gpu_available():
+        pytest.skip("a device is present")
+    assert backend.gpu_device_name() == "none"
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        backend.run_gpu_records(SimParams(), 10)
