@@ -15,6 +15,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/optphot_hip.h"
 #include "kernels.hpp"
@@ -76,6 +77,10 @@ struct Context {
   hipEvent_t ev[4] = {};
   DevBuf records[8];  // fate, nb, ns, t, x, y, z, path
   DevBuf partials, scratch, tally;
+  DevBuf batch_table, batch_offsets, batch_counts;
+  // Captured graph of the batched launch and the arguments it was built for.
+  hipGraphExec_t batch_exec = nullptr;
+  std::string batch_key;
 
   void init() {
     if (ready) return;
@@ -252,11 +257,13 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
   out.tally = T;
   out.partials = c.partials.as<uint32_t>();
   out.sums = scratch + kScratchSums;
-  out.counters = scratch + kScratchCounters;
+  // Branch-event counting is a separate kernel variant, used only on request.
+  out.counters = (counters || stats) ? scratch + kScratchCounters : nullptr;
   out.tally_on = true;
   out.records = false;
 
   launch_zero_u64(scratch, kScratchSums + 2 * static_cast<size_t>(grid), c.stream);
+  launch_zero_u64(c.tally.as<unsigned long long>(), static_cast<size_t>(T.words) + 2, c.stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.ev[0], c.stream));
   if (wavefront)
@@ -278,7 +285,7 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
   HIPCHK(hipStreamSynchronize(c.stream));
   tm.d2h_ms = ms_since(td);
   read_counters(c, counters);
-  if (stats) {
+  if (stats) {  // counted by the same launch (out.counters is set when stats is)
     optphot_step_counters cnt{};
     read_counters(c, &cnt);
     stats->launches = 1;
@@ -314,6 +321,23 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   s.steps_per_refill = t.steps_per_refill;
   s.queue_chunk = t.queue_chunk;
   return s;
+}
+
+void validate_batch(const optphot_sim_params* params, const uint64_t* photons, uint32_t n_events,
+                    uint32_t time_bins, float time_max, uint32_t* out, int method) {
+  if (!params || !photons || !out) throw InvalidArgument("batch: NULL argument");
+  if (n_events == 0) throw InvalidArgument("batch: n_events must be >= 1");
+  if (time_bins == 0 || time_bins > (1u << 20) || !(time_max > 0.0f))
+    throw InvalidArgument("batch: time_bins must be in [1, 2^20] and time_max > 0");
+  if (method != OPTPHOT_BATCH_WAVEFRONT && method != OPTPHOT_BATCH_PLAIN)
+    throw InvalidArgument("batch: unknown method");
+  for (uint32_t e = 0; e < n_events; ++e) {
+    validate_params(params + e);
+    if (params[e].seed != params[0].seed)
+      throw InvalidArgument("batch: every event must use the run seed params[0].seed");
+    if (photons[e] >= 0xffffffffull)
+      throw InvalidArgument("batch: an event's photon count must fit the u32 event tally");
+  }
 }
 
 template <class F>
@@ -397,7 +421,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records(
     LaunchOut lo{};
     lo.rec = rec;
     lo.records = true;
-    lo.counters = scratch + kScratchCounters;
+    lo.counters = nullptr;
+    (void)scratch;
     const DevParams P = to_dev(*p);
     for (uint64_t done = 0; done < n; done += cap) {
       const uint64_t m = std::min(cap, n - done);
@@ -517,6 +542,138 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records_event(
     st.issued_lane_steps = cnt.v[kCntIssued];
     tm.total_ms = ms_since(t0);
     if (stats) *stats = st;
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_batch(
+    const optphot_sim_params* params, const uint64_t* photons, uint32_t n_events,
+    uint64_t first_id, uint32_t time_bins, float time_max, uint32_t* event_tally, int method,
+    const optphot_wavefront_tuning* tuning, optphot_step_counters* counters,
+    optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_batch(params, photons, n_events, time_bins, time_max, event_tally, method);
+    optphot_wavefront_tuning tu{256, 4, 256, 0};
+    if (tuning) {
+      if (tuning->block_size) tu.block_size = tuning->block_size;
+      if (tuning->steps_per_refill) tu.steps_per_refill = tuning->steps_per_refill;
+      if (tuning->queue_chunk) tu.queue_chunk = tuning->queue_chunk;
+      if (tuning->blocks_per_cu) tu.blocks_per_cu = tuning->blocks_per_cu;
+    }
+    validate_block(tu.block_size, "batch");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+
+    // Host tables: per-event parameters and photon-id offsets.
+    std::vector<DevParams> table(n_events);
+    std::vector<unsigned long long> offsets(n_events + 1);
+    offsets[0] = 0;
+    for (uint32_t e = 0; e < n_events; ++e) {
+      table[e] = to_dev(params[e]);
+      offsets[e + 1] = offsets[e] + photons[e];
+    }
+    const uint64_t total = offsets[n_events];
+    const uint32_t words = 6u + time_bins;
+    const size_t count_words = static_cast<size_t>(n_events) * words;
+    const size_t zero_u64 = (count_words + 1) / 2;
+
+    const auto ta = std::chrono::steady_clock::now();
+    bool grew = c.batch_table.ensure(table.size() * sizeof(DevParams));
+    grew |= c.batch_offsets.ensure(offsets.size() * sizeof(unsigned long long));
+    grew |= c.batch_counts.ensure(zero_u64 * 8);
+    if (grew) {
+      tm.alloc_ms += ms_since(ta);
+      if (c.batch_exec) {  // buffers moved: the captured graph is stale
+        HIPCHK(hipGraphExecDestroy(c.batch_exec));
+        c.batch_exec = nullptr;
+        c.batch_key.clear();
+      }
+    }
+    const WavefrontShape s = [&] {
+      const int max_per_cu = std::max(1, batch_blocks_per_cu(tu.block_size));
+      const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
+      const uint64_t needed = (total + tu.block_size - 1) / tu.block_size;
+      WavefrontShape w{};
+      w.block = tu.block_size;
+      w.grid = static_cast<int>(std::max<uint64_t>(
+          1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
+      w.steps_per_refill = tu.steps_per_refill;
+      w.queue_chunk = tu.queue_chunk;
+      return w;
+    }();
+    if (s.queue_chunk < 64 || s.queue_chunk % 64 != 0 || s.steps_per_refill == 0)
+      throw InvalidArgument("batch: queue_chunk must be a multiple of 64, steps_per_refill >= 1");
+    unsigned long long* scratch = ensure_scratch(c, 1, tm.alloc_ms);
+
+    const auto th = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(c.batch_table.ptr, table.data(), table.size() * sizeof(DevParams),
+                          hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.batch_offsets.ptr, offsets.data(),
+                          offsets.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                          c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.h2d_ms = ms_since(th);
+
+    LaunchOut lo{};
+    lo.events = true;
+    lo.counters = counters ? scratch + kScratchCounters : nullptr;
+    lo.evt.counts = c.batch_counts.as<uint32_t>();
+    lo.evt.words = words;
+    lo.evt.time_bins = time_bins;
+    lo.evt.time_scale = static_cast<float>(time_bins) / time_max;  // transport.hpp:171
+    lo.evt.table = c.batch_table.as<DevParams>();
+    lo.evt.offsets = c.batch_offsets.as<unsigned long long>();
+    lo.evt.n_events = n_events;
+    const PhiloxKey key = key_of(params[0]);
+
+    if (method == OPTPHOT_BATCH_WAVEFRONT) {
+      const std::string gkey = std::to_string(n_events) + "/" + std::to_string(total) + "/" +
+                               std::to_string(first_id) + "/" + std::to_string(params[0].seed) +
+                               "/" + std::to_string(words) + "/" + std::to_string(lo.evt.time_scale) +
+                               "/" + std::to_string(s.block) + "/" + std::to_string(s.grid) + "/" +
+                               std::to_string(s.steps_per_refill) + "/" +
+                               std::to_string(s.queue_chunk);
+      if (!c.batch_exec || c.batch_key != gkey) {
+        if (c.batch_exec) HIPCHK(hipGraphExecDestroy(c.batch_exec));
+        c.batch_exec = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        launch_zero_u64(scratch, kScratchSums, c.stream);  // queue head + counters
+        launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64, c.stream);
+        launch_batch_wavefront(key, first_id, total, s, scratch + kScratchQueue, lo, c.stream);
+        HIPCHK(hipStreamEndCapture(c.stream, &g));
+        HIPCHK(hipGraphInstantiate(&c.batch_exec, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        c.batch_key = gkey;
+      }
+      HIPCHK(hipEventRecord(c.ev[0], c.stream));
+      HIPCHK(hipGraphLaunch(c.batch_exec, c.stream));
+      HIPCHK(hipEventRecord(c.ev[1], c.stream));
+    } else {
+      launch_zero_u64(scratch, kScratchSums, c.stream);
+      launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64, c.stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c.ev[0], c.stream));
+      for (uint32_t e = 0; e < n_events; ++e) {
+        if (photons[e] == 0) continue;
+        lo.evt.event = e;
+        launch_history_event(table[e], key, first_id + offsets[e], photons[e], tu.block_size, lo,
+                             c.stream);
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c.ev[1], c.stream));
+    }
+    tm.kernel_ms = c.elapsed(0, 1);
+    const auto td = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(event_tally, lo.evt.counts, count_words * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.d2h_ms = ms_since(td);
+    read_counters(c, counters);
+    tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
   });
 }

@@ -42,6 +42,21 @@ enum : int {
   kCntSlots = 16
 };
 
+// Per-event tallies of a batch: for event e, u32 words
+// [e * words .. e * words + 5) fate counts and
+// [e * words + 5 .. e * words + 5 + time_bins] arrival-time histogram with an
+// overflow bin (the reference's time_bin(), transport.hpp:170-173).
+struct DevEventTally {
+  uint32_t* counts;                 // [n_events][words]
+  uint32_t words;                   // 5 + time_bins + 1
+  uint32_t time_bins;
+  float time_scale;                 // time_bins / time_max
+  const DevParams* table;           // [n_events] per-event parameters (batched kernel)
+  const unsigned long long* offsets;  // [n_events + 1] prefix sums of photons per event
+  uint32_t n_events;
+  uint32_t event;                   // the single event of a plain per-event launch
+};
+
 // Output selection of a transport launch.
 struct LaunchOut {
   DevRecords rec;              // used when `records`
@@ -49,7 +64,8 @@ struct LaunchOut {
   uint32_t* partials;          // [grid][tally.words] block histograms
   unsigned long long* sums;    // [grid][2] block boundary / scatter sums
   unsigned long long* counters;  // [kCntSlots], accumulated
-  bool records, tally_on;
+  DevEventTally evt;           // used when `events`
+  bool records, tally_on, events;
 };
 
 struct WavefrontShape {
@@ -68,6 +84,20 @@ void launch_history(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64
 void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                       const WavefrontShape& shape, unsigned long long* queue, const LaunchOut& out,
                       hipStream_t stream);
+
+// Batched events: one persistent launch over the concatenated photon ids of
+// all events (event e owns ids first_id + offsets[e] .. first_id + offsets[e+1]),
+// per-lane event parameters from out.evt.table, per-event tallies.
+void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_total,
+                            const WavefrontShape& shape, unsigned long long* queue,
+                            const LaunchOut& out, hipStream_t stream);
+
+// Plain per-event launch: the history kernel over one event's photons with its
+// parameters as kernel arguments, tallied into out.evt.counts row out.evt.event.
+void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                          int block, const LaunchOut& out, hipStream_t stream);
+
+int batch_blocks_per_cu(int block);
 
 // Sums the per-block histograms and sums into the flat u64 tally layout.
 void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials,

@@ -26,6 +26,8 @@
 // writes them once to its row of a [grid][words] buffer, and a reduction
 // kernel sums the rows into the u64 tally. Integer sums: the result does not
 // depend on block size, grid size or scheduling.
+#include <stdexcept>
+
 #include "kernels.hpp"
 
 namespace ophip {
@@ -55,14 +57,21 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 
 __device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
 
-// Wave-uniform branch-event counters (kept in SGPRs).
+// Wave-uniform branch-event counters (kept in SGPRs). Compiled in only for
+// the launches that report them: 16 live 64-bit scalars otherwise push the
+// kernel's scalar registers into VGPR lanes.
+template <bool kOn>
 struct WaveCounters {
   uint64_t v[kCntSlots];
   __device__ __forceinline__ void clear() {
 #pragma unroll
     for (int i = 0; i < kCntSlots; ++i) v[i] = 0;
   }
+  __device__ __forceinline__ void add(int slot, uint64_t x) {
+    if (kOn) v[slot] += x;
+  }
   __device__ __forceinline__ void add_step(uint64_t active, const StepInfo& s) {
+    if (!kOn) return;
     v[kCntUseful] += popc(active);
     v[kCntIssued] += 64;
     v[kCntAbsorb] += popc(__ballot(s.absorb));
@@ -74,7 +83,7 @@ struct WaveCounters {
     v[kCntLambert] += popc(__ballot(s.lambert_reflect));
   }
   __device__ __forceinline__ void flush(unsigned long long* dst) const {
-    if (lane_id() == 0) {
+    if (kOn && lane_id() == 0) {
 #pragma unroll
       for (int i = 0; i < kCntSlots; ++i)
         if (v[i]) atomicAdd(dst + i, static_cast<unsigned long long>(v[i]));
@@ -86,11 +95,18 @@ __device__ __forceinline__ void zero_lds(uint32_t* lds, uint32_t words) {
   for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) lds[w] = 0u;
 }
 
-// Output of one finished photon: its record at `idx` and/or its tally entries.
-template <bool kRec, bool kTally>
+// Output of one finished photon: its record at `idx`, its block-tally entries
+// and/or its event-tally entries (event `e`, global u32 atomics).
+template <bool kRec, bool kTally, bool kEvt = false>
 __device__ __forceinline__ void finish(const DevParams& P, const LaunchOut& out, uint32_t* lds,
                                        uint64_t idx, const Photon& ph, uint32_t fate,
-                                       uint64_t& snb, uint64_t& sns) {
+                                       uint64_t& snb, uint64_t& sns, uint32_t e = 0) {
+  if (kEvt) {
+    uint32_t* row = out.evt.counts + static_cast<size_t>(e) * out.evt.words;
+    atomicAdd(row + fate, 1u);
+    if (fate == kDetected)
+      atomicAdd(row + 5u + time_bin(ph.t, out.evt.time_scale, out.evt.time_bins), 1u);
+  }
   if (kRec) {
     out.rec.fate[idx] = static_cast<uint8_t>(fate);
     out.rec.nb[idx] = ph.nb;
@@ -115,9 +131,9 @@ __device__ __forceinline__ void finish(const DevParams& P, const LaunchOut& out,
   }
 }
 
-template <bool kTally>
+template <bool kTally, bool kCount>
 __device__ __forceinline__ void flush_block(const LaunchOut& out, uint32_t* lds, uint64_t snb,
-                                            uint64_t sns, const WaveCounters& wc) {
+                                            uint64_t sns, const WaveCounters<kCount>& wc) {
   wc.flush(out.counters);
   if (kTally) {
     const uint64_t bnb = wave_sum_u64(snb), bns = wave_sum_u64(sns);
@@ -134,7 +150,7 @@ __device__ __forceinline__ void flush_block(const LaunchOut& out, uint32_t* lds,
 // ---------------------------------------------------------------------------
 // History-based kernel. Records: the grid covers n, lane i <-> photon i.
 // Tally: waves stride over the photons in steps of the whole grid.
-template <bool kRec, bool kTally>
+template <bool kRec, bool kTally, bool kEvt, bool kCount>
 __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey key,
                                                        uint64_t first_id, uint64_t n,
                                                        LaunchOut out) {
@@ -146,7 +162,7 @@ __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey ke
   const uint32_t lane = lane_id();
   const uint64_t wave0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x - lane);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  WaveCounters wc;
+  WaveCounters<kCount> wc;
   wc.clear();
   uint64_t snb = 0, sns = 0;
 
@@ -160,7 +176,7 @@ __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey ke
       rng.start(first_id + i, key, 0u);
       emit(P, rng, ph);
     }
-    wc.v[kCntEmitted] += popc(__ballot(alive));
+    if (kCount) wc.add(kCntEmitted, popc(__ballot(alive)));
     const bool valid = alive;
     alive = alive && P.max_steps > 0u;
     uint32_t done = 0;
@@ -177,28 +193,47 @@ __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey ke
       }
       wc.add_step(act, info);
     }
-    wc.v[kCntDetected] += popc(__ballot(valid && fate == kDetected));
-    if (valid) finish<kRec, kTally>(P, out, lds, i, ph, fate, snb, sns);
+    if (kCount) wc.add(kCntDetected, popc(__ballot(valid && fate == kDetected)));
+    if (valid) finish<kRec, kTally, kEvt>(P, out, lds, i, ph, fate, snb, sns, out.evt.event);
   }
   if (kTally) __syncthreads();
-  flush_block<kTally>(out, lds, snb, sns, wc);
+  flush_block<kTally, kCount>(out, lds, snb, sns, wc);
 }
 
 // ---------------------------------------------------------------------------
-// Persistent wavefront kernel (see the file comment).
-template <bool kRec, bool kTally>
-__global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P, PhiloxKey key,
+// Event of batch photon `idx`: the largest e with offsets[e] <= idx (events
+// with no photons have equal offsets and are never chosen).
+__device__ __forceinline__ uint32_t find_event(const unsigned long long* offsets, uint32_t n_events,
+                                               uint64_t idx) {
+  uint32_t lo = 0, hi = n_events;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= idx)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Persistent wavefront kernel (see the file comment). With kBatch the lanes
+// take photons of many events: each lane looks up its photon's event when it
+// is refilled and carries that event's parameters in registers.
+template <bool kRec, bool kTally, bool kBatch, bool kCount>
+__global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey key,
                                                          uint64_t first_id, uint64_t n,
                                                          uint32_t steps_per_refill, uint32_t chunk,
                                                          unsigned long long* queue,
                                                          LaunchOut out) {
+  DevParams P = P0;
+  uint32_t ev = 0;
   extern __shared__ uint32_t lds[];
   if (kTally) {
     zero_lds(lds, out.tally.words);
     __syncthreads();
   }
   const uint32_t lane = lane_id();
-  WaveCounters wc;
+  WaveCounters<kCount> wc;
   wc.clear();
   uint64_t snb = 0, sns = 0;
 
@@ -239,6 +274,10 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P, PhiloxKey 
       bool fresh = false;
       if (!alive && r < t1 + t2) {
         idx = r < t1 ? b1 + r : b2 + (r - t1);
+        if (kBatch) {
+          ev = find_event(out.evt.offsets, out.evt.n_events, idx);
+          P = out.evt.table[ev];
+        }
         rng.start(first_id + idx, key, 0u);
         emit(P, rng, ph);
         done = 0;
@@ -246,9 +285,9 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P, PhiloxKey 
         alive = P.max_steps > 0u;
         fresh = true;
       }
-      wc.v[kCntEmitted] += t1 + t2;
+      wc.add(kCntEmitted, t1 + t2);
       if (fresh && !alive) {  // max_steps == 0: finished at emission
-        finish<kRec, kTally>(P, out, lds, idx, ph, fate, snb, sns);
+        finish<kRec, kTally, kBatch>(P, out, lds, idx, ph, fate, snb, sns, ev);
       }
     }
     if (__ballot(alive) == 0) {
@@ -273,27 +312,30 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P, PhiloxKey 
         }
       }
       wc.add_step(act, info);
-      wc.v[kCntDetected] += popc(__ballot(finished && fate == kDetected));
-      if (finished) finish<kRec, kTally>(P, out, lds, idx, ph, fate, snb, sns);
+      if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+      if (finished) finish<kRec, kTally, kBatch>(P, out, lds, idx, ph, fate, snb, sns, ev);
     }
   }
   if (kTally) __syncthreads();
-  flush_block<kTally>(out, lds, snb, sns, wc);
+  flush_block<kTally, kCount>(out, lds, snb, sns, wc);
 }
 
+// Column sums of the [blocks][words] partial histograms (and the [blocks][2]
+// sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
+// every gridDim.y-th block row, so enough waves are in flight to stream the
+// partials at HBM rate; each column slice adds its sum with one atomic.
 __global__ void tally_reduce_kernel(uint32_t words, int blocks, const uint32_t* partials,
                                     const unsigned long long* sums, unsigned long long* tally) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long s = 0;
   if (w < words) {
-    unsigned long long s = 0;
-    for (int b = 0; b < blocks; ++b) s += partials[static_cast<size_t>(b) * words + w];
-    tally[w] = s;
+    for (int b = blockIdx.y; b < blocks; b += gridDim.y)
+      s += partials[static_cast<size_t>(b) * words + w];
   } else if (w < words + 2u) {
     const uint32_t j = w - words;
-    unsigned long long s = 0;
-    for (int b = 0; b < blocks; ++b) s += sums[2 * b + j];
-    tally[w] = s;
+    for (int b = blockIdx.y; b < blocks; b += gridDim.y) s += sums[2 * b + j];
   }
+  if (w < words + 2u && s != 0) atomicAdd(tally + w, s);
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p, uint64_t n) {
@@ -303,21 +345,21 @@ __global__ void zero_u64_kernel(unsigned long long* p, uint64_t n) {
     p[i] = 0ull;
 }
 
-template <bool R, bool T>
+template <bool R, bool T, bool E, bool C>
 void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n, int block,
                       int grid, const LaunchOut& out, hipStream_t stream) {
   const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
-  hipLaunchKernelGGL((history_kernel<R, T>), dim3(grid), dim3(block), lds, stream, P, key,
+  hipLaunchKernelGGL((history_kernel<R, T, E, C>), dim3(grid), dim3(block), lds, stream, P, key,
                      first_id, n, out);
 }
 
-template <bool R, bool T>
+template <bool R, bool T, bool B, bool C>
 void launch_wavefront_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                         const WavefrontShape& s, unsigned long long* queue, const LaunchOut& out,
                         hipStream_t stream) {
   const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
-  hipLaunchKernelGGL((wavefront_kernel<R, T>), dim3(s.grid), dim3(s.block), lds, stream, P, key,
-                     first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+  hipLaunchKernelGGL((wavefront_kernel<R, T, B, C>), dim3(s.grid), dim3(s.block), lds, stream, P,
+                     key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
 }
 
 template <class K>
@@ -329,29 +371,51 @@ int blocks_per_cu(K kernel, int block, size_t lds) {
 
 }  // namespace
 
+// Instantiated variants: records (no counters), tally with / without
+// counters, and the plain per-event launches of a batch.
 void launch_history(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n, int block,
                     int grid, const LaunchOut& out, hipStream_t stream) {
-  if (out.records && out.tally_on)
-    launch_history_t<true, true>(P, key, first_id, n, block, grid, out, stream);
-  else if (out.records)
-    launch_history_t<true, false>(P, key, first_id, n, block, grid, out, stream);
-  else if (out.tally_on)
-    launch_history_t<false, true>(P, key, first_id, n, block, grid, out, stream);
+  const bool count = out.counters != nullptr;
+  if (out.records && !out.tally_on)
+    launch_history_t<true, false, false, false>(P, key, first_id, n, block, grid, out, stream);
+  else if (out.tally_on && !out.records && count)
+    launch_history_t<false, true, false, true>(P, key, first_id, n, block, grid, out, stream);
+  else if (out.tally_on && !out.records)
+    launch_history_t<false, true, false, false>(P, key, first_id, n, block, grid, out, stream);
   else
-    launch_history_t<false, false>(P, key, first_id, n, block, grid, out, stream);
+    throw std::invalid_argument("launch_history: unsupported output combination");
 }
 
 void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                       const WavefrontShape& shape, unsigned long long* queue, const LaunchOut& out,
                       hipStream_t stream) {
-  if (out.records && out.tally_on)
-    launch_wavefront_t<true, true>(P, key, first_id, n, shape, queue, out, stream);
-  else if (out.records)
-    launch_wavefront_t<true, false>(P, key, first_id, n, shape, queue, out, stream);
-  else if (out.tally_on)
-    launch_wavefront_t<false, true>(P, key, first_id, n, shape, queue, out, stream);
+  const bool count = out.counters != nullptr;
+  if (out.records && !out.tally_on)
+    launch_wavefront_t<true, false, false, true>(P, key, first_id, n, shape, queue, out, stream);
+  else if (out.tally_on && !out.records && count)
+    launch_wavefront_t<false, true, false, true>(P, key, first_id, n, shape, queue, out, stream);
+  else if (out.tally_on && !out.records)
+    launch_wavefront_t<false, true, false, false>(P, key, first_id, n, shape, queue, out, stream);
   else
-    launch_wavefront_t<false, false>(P, key, first_id, n, shape, queue, out, stream);
+    throw std::invalid_argument("launch_wavefront: unsupported output combination");
+}
+
+void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_total,
+                            const WavefrontShape& shape, unsigned long long* queue,
+                            const LaunchOut& out, hipStream_t stream) {
+  if (out.counters)
+    launch_wavefront_t<false, false, true, true>(DevParams{}, key, first_id, n_total, shape, queue,
+                                                 out, stream);
+  else
+    launch_wavefront_t<false, false, true, false>(DevParams{}, key, first_id, n_total, shape,
+                                                  queue, out, stream);
+}
+
+void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                          int block, const LaunchOut& out, hipStream_t stream) {
+  const int grid = static_cast<int>((n + block - 1) / block);
+  if (grid > 0)
+    launch_history_t<false, false, true, false>(P, key, first_id, n, block, grid, out, stream);
 }
 
 void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials,
@@ -359,8 +423,9 @@ void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials
                          hipStream_t stream) {
   const uint32_t total = t.words + 2u;
   const int threads = 256;
-  hipLaunchKernelGGL(tally_reduce_kernel, dim3((total + threads - 1) / threads), dim3(threads), 0,
-                     stream, t.words, blocks, partials, sums, tally);
+  const int slices = blocks < 64 ? blocks : 64;
+  hipLaunchKernelGGL(tally_reduce_kernel, dim3((total + threads - 1) / threads, slices),
+                     dim3(threads), 0, stream, t.words, blocks, partials, sums, tally);
 }
 
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {
@@ -372,15 +437,17 @@ void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {
 }
 
 int history_blocks_per_cu(bool tally, int block, size_t lds) {
-  return tally ? blocks_per_cu(history_kernel<false, true>, block, lds)
-               : blocks_per_cu(history_kernel<true, false>, block, lds);
+  return tally ? blocks_per_cu(history_kernel<false, true, false, false>, block, lds)
+               : blocks_per_cu(history_kernel<true, false, false, false>, block, lds);
 }
 
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds) {
-  if (records && tally) return blocks_per_cu(wavefront_kernel<true, true>, block, lds);
-  if (records) return blocks_per_cu(wavefront_kernel<true, false>, block, lds);
-  if (tally) return blocks_per_cu(wavefront_kernel<false, true>, block, lds);
-  return blocks_per_cu(wavefront_kernel<false, false>, block, lds);
+  if (records) return blocks_per_cu(wavefront_kernel<true, false, false, true>, block, lds);
+  return blocks_per_cu(wavefront_kernel<false, true, false, false>, block, lds);
+}
+
+int batch_blocks_per_cu(int block) {
+  return blocks_per_cu(wavefront_kernel<false, false, true, false>, block, 0);
 }
 
 }  // namespace ophip

@@ -166,6 +166,27 @@ int optphot_run_gpu_tally_counters(const optphot_sim_params* p, const optphot_ta
                                    uint64_t n, uint64_t* tally, int block_size, uint64_t first_id,
                                    optphot_step_counters* counters, optphot_run_timing* timing);
 
+/* Batched events. Event e has parameters params[e] and photons[e] photons,
+ * with Philox photon ids first_id + offset_e .. first_id + offset_e +
+ * photons[e] - 1, offset_e = photons[0] + ... + photons[e-1]; every event
+ * uses the run seed params[0].seed (all seeds must be equal). Event e's
+ * tally is the u32 row event_tally[e * (6 + time_bins) ...]: 5 fate counts,
+ * then the arrival-time histogram of detected photons on [0, time_max) with
+ * an overflow bin. So event e equals the reference's
+ * run_cpu_tally(params[e], {time_bins, time_max}, photons[e], ..., first_id +
+ * offset_e) (backend.hpp:78-79) restricted to those words.
+ *   OPTPHOT_BATCH_WAVEFRONT: one persistent launch for all events, captured in
+ *     a hipGraph and replayed by later calls of the same shape;
+ *   OPTPHOT_BATCH_PLAIN: one history-kernel launch per event, in order.
+ * tuning / counters / timing may be NULL. */
+#define OPTPHOT_BATCH_WAVEFRONT 0
+#define OPTPHOT_BATCH_PLAIN 1
+int optphot_run_gpu_batch(const optphot_sim_params* params, const uint64_t* photons,
+                          uint32_t n_events, uint64_t first_id, uint32_t time_bins,
+                          float time_max, uint32_t* event_tally, int method,
+                          const optphot_wavefront_tuning* tuning,
+                          optphot_step_counters* counters, optphot_run_timing* timing);
+
 /* Number of compute units and the kernels' register / occupancy facts,
  * written as "key=value" lines into buf (diagnostics). */
 int optphot_device_info(char* buf, size_t len);
