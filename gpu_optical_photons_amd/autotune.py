@@ -68,7 +68,7 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS, 
         tu = {"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0}
         best = math.inf
         for _ in range(reps):
-            t, tm, st, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tu)
+            t, tm, st, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)
             best = min(best, tm.kernel_ms)
         flat = t.flat()
         if ref is None:

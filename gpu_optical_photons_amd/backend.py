@@ -231,7 +231,8 @@ def run_gpu_tally(p: SimParams, c: TallyConfig, n: int, block_size: int = 256,
     cnt = StepCounters()
     rc = _lib.optphot_run_gpu_tally_counters(ctypes.byref(p), ctypes.byref(c), n,
                                              flat.ctypes.data, block_size, first_id,
-                                             ctypes.byref(cnt), ctypes.byref(tm))
+                                             ctypes.byref(cnt) if counters else None,
+                                             ctypes.byref(tm))
     _check(rc)
     t = Tally.from_flat(flat, c)
     return (t, tm, cnt) if counters else (t, tm)
@@ -253,16 +254,23 @@ def run_gpu_records_event(p: SimParams, n: int, steps_per_launch: int, block_siz
 
 
 def run_gpu_tally_wavefront(p: SimParams, c: TallyConfig, n: int, first_id: int = 0,
-                            tuning: WavefrontTuning | dict | None = None):
+                            tuning: WavefrontTuning | dict | None = None, counters: bool = True):
     """Tally by the persistent wavefront kernel (new; no reference counterpart).
-    Returns (Tally, RunTiming, EventStats, StepCounters)."""
+    Returns (Tally, RunTiming, EventStats, StepCounters). With counters=False
+    the launch uses the kernel variant without branch-event counting (the
+    one to time); EventStats then carries only launches and kernels_ms and
+    the StepCounters are zero."""
     if isinstance(tuning, dict):
         tuning = WavefrontTuning(**tuning)
     flat = np.zeros(c.words, np.uint64)
     tm, st, cnt = RunTiming(), EventStats(), StepCounters()
     rc = _lib.optphot_run_gpu_tally_wavefront(
         ctypes.byref(p), ctypes.byref(c), n, flat.ctypes.data, first_id,
-        ctypes.byref(tuning) if tuning is not None else None, ctypes.byref(st), ctypes.byref(cnt),
+        ctypes.byref(tuning) if tuning is not None else None,
+        ctypes.byref(st) if counters else None, ctypes.byref(cnt) if counters else None,
         ctypes.byref(tm))
     _check(rc)
+    if not counters:
+        st.launches = 1
+        st.kernels_ms = tm.kernel_ms
     return Tally.from_flat(flat, c), tm, st, cnt

@@ -81,6 +81,11 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c2, uint32_t c3, 
     x2 = y2;
     k0 += 0x9E3779B9u;  // Weyl key schedule
     k1 += 0xBB67AE85u;
+    // Keep the schedule as two scalar adds per round: left visible, the
+    // compiler hoists all 20 round keys out of the transport loop, and the
+    // scalar registers they occupy spill into VGPR lanes (v_readlane in the
+    // hot loop).
+    asm volatile("" : "+s"(k0), "+s"(k1));
   }
   o0 = x0;
   o1 = x1;
@@ -277,8 +282,12 @@ __device__ __forceinline__ void emit(const DevParams& P, Stream& rng, Photon& ph
 // One transport step (transport.hpp:71-145). Returns true while the photon
 // is alive; otherwise *fate is set. The caller must have called
 // rng.refill() so that pos <= 3.
+// kSurf >= 0 fixes the surface model at compile time (a launch whose events
+// all share it); kSurf < 0 reads it from P.
+template <int kSurf = -1>
 __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph, uint32_t& fate,
                                      StepInfo& info) {
+  const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
   // --- two exponential distances, always drawn (transport.hpp:76-77) -------
   const float u_abs = to_unit(rng.peek<0>());
   const float u_sca = to_unit(rng.peek<1>());
@@ -342,10 +351,10 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
 
   const bool readout = boundary && face == kReadoutFace;
   const bool side = boundary && face != kReadoutFace;
-  const bool polished = side && P.surface == kPolished;
-  const bool black = side && P.surface == kBlack;
-  const bool refl = side && P.surface >= kSpecular;
-  const bool lambert = P.surface == kLambertian;
+  const bool polished = side && surface == kPolished;
+  const bool black = side && surface == kBlack;
+  const bool refl = side && surface >= kSpecular;
+  const bool lambert = surface == kLambertian;
 
   // --- Fresnel (fresnel.hpp:30-46), once for readout and polished lanes -----
   float prob = P.reflectivity;

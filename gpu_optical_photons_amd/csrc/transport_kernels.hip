@@ -219,7 +219,7 @@ __device__ __forceinline__ uint32_t find_event(const unsigned long long* offsets
 // Persistent wavefront kernel (see the file comment). With kBatch the lanes
 // take photons of many events: each lane looks up its photon's event when it
 // is refilled and carries that event's parameters in registers.
-template <bool kRec, bool kTally, bool kBatch, bool kCount>
+template <bool kRec, bool kTally, bool kBatch, bool kCount, int kSurf = -1>
 __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey key,
                                                          uint64_t first_id, uint64_t n,
                                                          uint32_t steps_per_refill, uint32_t chunk,
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
       bool finished = false;
       if (alive) {
         rng.refill(key);
-        const bool still = step(P, rng, ph, fate, info);
+        const bool still = step<kSurf>(P, rng, ph, fate, info);
         ++done;
         if (!still || done >= P.max_steps) {
           if (still) fate = kMaxSteps;
@@ -353,13 +353,37 @@ void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
                      first_id, n, out);
 }
 
-template <bool R, bool T, bool B, bool C>
+template <bool R, bool T, bool B, bool C, int S = -1>
 void launch_wavefront_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                         const WavefrontShape& s, unsigned long long* queue, const LaunchOut& out,
                         hipStream_t stream) {
   const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
-  hipLaunchKernelGGL((wavefront_kernel<R, T, B, C>), dim3(s.grid), dim3(s.block), lds, stream, P,
-                     key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+  hipLaunchKernelGGL((wavefront_kernel<R, T, B, C, S>), dim3(s.grid), dim3(s.block), lds, stream,
+                     P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+}
+
+// Tally launches specialised on the run's surface model.
+template <bool C>
+void launch_wavefront_tally(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                            const WavefrontShape& s, unsigned long long* queue,
+                            const LaunchOut& out, hipStream_t stream) {
+  switch (P.surface) {
+    case kPolished:
+      launch_wavefront_t<false, true, false, C, kPolished>(P, key, first_id, n, s, queue, out,
+                                                           stream);
+      break;
+    case kBlack:
+      launch_wavefront_t<false, true, false, C, kBlack>(P, key, first_id, n, s, queue, out, stream);
+      break;
+    case kSpecular:
+      launch_wavefront_t<false, true, false, C, kSpecular>(P, key, first_id, n, s, queue, out,
+                                                           stream);
+      break;
+    default:
+      launch_wavefront_t<false, true, false, C, kLambertian>(P, key, first_id, n, s, queue, out,
+                                                             stream);
+      break;
+  }
 }
 
 template <class K>
@@ -393,9 +417,9 @@ void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
   if (out.records && !out.tally_on)
     launch_wavefront_t<true, false, false, true>(P, key, first_id, n, shape, queue, out, stream);
   else if (out.tally_on && !out.records && count)
-    launch_wavefront_t<false, true, false, true>(P, key, first_id, n, shape, queue, out, stream);
+    launch_wavefront_tally<true>(P, key, first_id, n, shape, queue, out, stream);
   else if (out.tally_on && !out.records)
-    launch_wavefront_t<false, true, false, false>(P, key, first_id, n, shape, queue, out, stream);
+    launch_wavefront_tally<false>(P, key, first_id, n, shape, queue, out, stream);
   else
     throw std::invalid_argument("launch_wavefront: unsupported output combination");
 }
@@ -443,7 +467,7 @@ int history_blocks_per_cu(bool tally, int block, size_t lds) {
 
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds) {
   if (records) return blocks_per_cu(wavefront_kernel<true, false, false, true>, block, lds);
-  return blocks_per_cu(wavefront_kernel<false, true, false, false>, block, lds);
+  return blocks_per_cu(wavefront_kernel<false, true, false, false, kLambertian>, block, lds);
 }
 
 int batch_blocks_per_cu(int block) {
