@@ -1,0 +1,190 @@
+"""Differentiation workloads: gradient checks and batched calibration.
+
+Single geometry: the plain-small box (50 mm cube, polished faces to n_out
+1.0, +z readout, point source at the centre, tau 4 ns) at the nominal
+theta = (abs 1000 mm, scat 300 mm, R 0.97, n_det 1.5, n_scint 1.63) -- a
+finite scattering length so that d/d scat_length is not identically zero;
+R does not enter with polished faces, so its gradient must vanish. Loss: the
+chi-square of adjoint.chi2_loss between the simulated 256-bin arrival-time
+histogram on [0, 50) ns (plus overflow and the detected fraction) and a
+target generated at (1500, 400, 0.95, 1.45, 1.60) with a different seed.
+
+Batched calibration: parameter sets with the long-histories geometry
+(100 x 100 x 50 mm, Lambertian walls, Rayleigh, volume source), hidden
+(abs_length, R, n_det) drawn from U[300, 3000] x U[0.85, 0.99] x U[1.4, 1.6],
+targets generated under a held-out seed; each set fits its three parameters
+by Adam on normalised coordinates, one forward (batched tally) + adjoint
+pass per iteration.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import adjoint
+from .batch import EventBatch
+from .params import SimParams, Source, Surface
+
+C4_NOMINAL = np.array([1000.0, 300.0, 0.97, 1.5, 1.63])
+C4_TARGET = np.array([1500.0, 400.0, 0.95, 1.45, 1.60])
+C4_BINS, C4_TMAX = 256, 50.0
+
+
+def c4_template(seed: int = 1) -> SimParams:
+    return SimParams(hx=25.0, hy=25.0, hz=25.0, n_out=1.0, surface=Surface.POLISHED,
+                     source=Source.POINT, tau=4.0, max_steps=10000, seed=seed)
+
+
+def c4_problem(n: int, n_target: int | None = None, seed: int = 1, target_seed: int = 777):
+    """(template, target histogram [bins+1]) for the single-geometry loss."""
+    n_target = n_target or n
+    tgt = adjoint.with_theta(c4_template(target_seed), C4_TARGET)
+    T, _ = adjoint.histograms(EventBatch([tgt], np.array([n_target], np.uint64)), C4_BINS, C4_TMAX)
+    return c4_template(seed), T[0]
+
+
+def loss_and_grad(template, theta, target, n, n_target, first_id=0):
+    """C4 loss at theta and its adjoint gradient (same photons)."""
+    p = adjoint.with_theta(template, theta)
+    b = EventBatch([p], np.array([n], np.uint64))
+    h, tf = adjoint.histograms(b, C4_BINS, C4_TMAX, first_id)
+    h = h[0]
+    f, F = h.sum(), target.sum()
+    wk = 1.0 / (target + 1.0 / n_target)
+    wf = 1.0 / (F * (1.0 - F) + 1.0 / n_target)
+    loss = float(((h - target) ** 2 * wk).sum() + (f - F) ** 2 * wf)
+    dl_dh = 2.0 * (h - target) * wk + 2.0 * (f - F) * wf
+    g, ta = adjoint.run_gpu_adjoint(b, dl_dh[None, :], C4_BINS, C4_TMAX, first_id)
+    return loss, g[0], tf.kernel_ms, ta.kernel_ms, dl_dh
+
+
+def loss_at(template, theta, target, n, n_target, first_id=0):
+    p = adjoint.with_theta(template, theta)
+    h, _ = adjoint.histograms(EventBatch([p], np.array([n], np.uint64)), C4_BINS, C4_TMAX,
+                              first_id)
+    return float(adjoint.chi2_loss(h[0], target, n, n_target)), h[0]
+
+
+def fd_gradient(template, theta, target, n, n_target, rel_step=1e-3, parts=4, dl_dh=None):
+    """Central finite differences with common random numbers (same seed and
+    photon ids at theta +- h), in float64 from integer tallies. Each of
+    `parts` disjoint photon-id ranges gives one estimate; returns mean and
+    standard error. With dl_dh given, differentiates the linearised loss
+    sum_k dl_dh[k] h_k (the quantity the adjoint returns), which removes the
+    O(h^2) curvature of the chi-square from the comparison."""
+    m = n // parts
+    est = np.zeros((parts, 5))
+    for j in range(5):
+        step = rel_step * abs(theta[j])
+        tp, tm_ = theta.copy(), theta.copy()
+        tp[j] += step
+        tm_[j] -= step
+        # the step actually taken, after float32 rounding of the parameters
+        dp = np.float32(tp[j]) - np.float32(tm_[j])
+        for k in range(parts):
+            lp, hp = loss_at(template, tp, target, m, n_target, first_id=k * m)
+            lm, hm = loss_at(template, tm_, target, m, n_target, first_id=k * m)
+            if dl_dh is not None:
+                est[k, j] = float(dl_dh @ (hp - hm)) / float(dp)
+            else:
+                est[k, j] = (lp - lm) / float(dp)
+    return est.mean(axis=0), est.std(axis=0, ddof=1) / math.sqrt(parts)
+
+
+def lr_gradient(template, theta, target, n, n_target, parts=4, dl_dh=None):
+    """Adjoint gradient on `parts` disjoint id ranges: mean and standard error.
+    With dl_dh given, that fixed weight vector is used for every part."""
+    m = n // parts
+    est = np.zeros((parts, 5))
+    for k in range(parts):
+        if dl_dh is None:
+            _, g, _, _, _ = loss_and_grad(template, theta, target, m, n_target, first_id=k * m)
+        else:
+            p = adjoint.with_theta(template, theta)
+            b = EventBatch([p], np.array([m], np.uint64))
+            g, _ = adjoint.run_gpu_adjoint(b, dl_dh[None, :], C4_BINS, C4_TMAX, k * m)
+            g = g[0]
+        est[k] = g
+    return est.mean(axis=0), est.std(axis=0, ddof=1) / math.sqrt(parts)
+
+
+# ---------------------------------------------------------------------------
+# Batched calibration
+
+LO = np.array([300.0, 0.85, 1.4])
+HI = np.array([3000.0, 0.99, 1.6])
+FIT = (0, 2, 3)  # theta indices fitted: abs_length, reflectivity, n_det
+
+
+def c5_template(seed: int) -> SimParams:
+    return SimParams(hx=50.0, hy=50.0, hz=25.0, n_scint=1.63, n_out=1.0, scat_length=300.0,
+                     surface=Surface.LAMBERTIAN, source=Source.VOLUME, tau=4.0,
+                     max_steps=10000, seed=seed)
+
+
+def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=0.08,
+              draw_seed=5, seed=1, target_seed=99, target_photons=None, log=None):
+    """Fits (abs_length, R, n_det) of every set to its own target histogram.
+    Returns a dict with hidden / fitted parameters, errors and timings."""
+    rng = np.random.default_rng(draw_seed)
+    hidden = LO + (HI - LO) * rng.random((n_sets, 3))
+    target_photons = target_photons or photons
+    base = np.array([2000.0, 300.0, 0.97, 1.5, 1.63])
+
+    def theta_of(x):  # x: normalised [n_sets, 3] in [0, 1]
+        th = np.tile(base, (n_sets, 1))
+        th[:, list(FIT)] = LO + (HI - LO) * x
+        return th
+
+    # targets at the hidden parameters, held-out seed
+    tpl_t = c5_template(target_seed)
+    tb = EventBatch([adjoint.with_theta(tpl_t, th) for th in theta_of((hidden - LO) / (HI - LO))],
+                    np.full(n_sets, target_photons, np.uint64))
+    T, _ = adjoint.histograms(tb, bins, tmax)
+
+    x = np.full((n_sets, 3), 0.5)
+    m = np.zeros_like(x)
+    v = np.zeros_like(x)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    tpl = c5_template(seed)
+    fwd_ms = adj_ms = 0.0
+    wall0 = time.perf_counter()
+    losses = []
+    for it in range(1, iters + 1):
+        th = theta_of(x)
+        batch = EventBatch([adjoint.with_theta(tpl, r) for r in th],
+                           np.full(n_sets, photons, np.uint64))
+        h, tf = adjoint.histograms(batch, bins, tmax)
+        f, F = h.sum(1, keepdims=True), T.sum(1, keepdims=True)
+        wk = 1.0 / (T + 1.0 / target_photons)
+        wf = 1.0 / (F * (1.0 - F) + 1.0 / target_photons)
+        loss = ((h - T) ** 2 * wk).sum(1) + ((f - F) ** 2 * wf)[:, 0]
+        dl_dh = 2.0 * (h - T) * wk + 2.0 * (f - F) * wf
+        g, ta = adjoint.run_gpu_adjoint(batch, dl_dh, bins, tmax)
+        fwd_ms += tf.kernel_ms
+        adj_ms += ta.kernel_ms
+        gx = g[:, list(FIT)] * (HI - LO)  # chain rule to normalised coordinates
+        m = b1 * m + (1 - b1) * gx
+        v = b2 * v + (1 - b2) * gx * gx
+        x = np.clip(x - lr * (m / (1 - b1 ** it)) / (np.sqrt(v / (1 - b2 ** it)) + eps), 0.0, 1.0)
+        losses.append(float(np.median(loss)))
+        if log:
+            log(f"iter {it:3d}: median loss {losses[-1]:.4e}")
+    wall = time.perf_counter() - wall0
+    fitted = LO + (HI - LO) * x
+    rel_err = np.abs(fitted - hidden) / hidden
+    total = n_sets * photons * iters
+    return {
+        "n_sets": n_sets, "photons_per_set": photons, "iterations": iters,
+        "hidden": hidden, "fitted": fitted,
+        "median_rel_error": dict(zip(("abs_length", "reflectivity", "n_det"),
+                                     np.median(rel_err, axis=0).tolist())),
+        "p90_rel_error": dict(zip(("abs_length", "reflectivity", "n_det"),
+                                  np.percentile(rel_err, 90, axis=0).tolist())),
+        "losses": losses,
+        "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "wall_s": wall,
+        "forward_photons_per_s": total / (fwd_ms * 1e-3),
+        "fwd_adj_photons_per_s": total / ((fwd_ms + adj_ms) * 1e-3),
+    }

@@ -78,6 +78,7 @@ struct Context {
   DevBuf records[8];  // fate, nb, ns, t, x, y, z, path
   DevBuf partials, scratch, tally;
   DevBuf batch_table, batch_offsets, batch_counts;
+  DevBuf adj_weights, adj_grad, adj_queue;
   // Captured graph of the batched launch and the arguments it was built for.
   hipGraphExec_t batch_exec = nullptr;
   std::string batch_key;
@@ -673,6 +674,103 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     HIPCHK(hipStreamSynchronize(c.stream));
     tm.d2h_ms = ms_since(td);
     read_counters(c, counters);
+    tm.total_ms = ms_since(t0);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_adjoint(
+    const optphot_sim_params* params, const uint64_t* photons, uint32_t n_events,
+    uint64_t first_id, uint32_t time_bins, float time_max, const double* weights, double* grad,
+    const optphot_wavefront_tuning* tuning, optphot_run_timing* timing) {
+  return guarded([&] {
+    if (!weights || !grad) throw InvalidArgument("adjoint: NULL weights / grad");
+    validate_batch(params, photons, n_events, time_bins, time_max,
+                   reinterpret_cast<uint32_t*>(grad), OPTPHOT_BATCH_WAVEFRONT);
+    optphot_wavefront_tuning tu{256, 2, 256, 0};
+    if (tuning) {
+      if (tuning->block_size) tu.block_size = tuning->block_size;
+      if (tuning->steps_per_refill) tu.steps_per_refill = tuning->steps_per_refill;
+      if (tuning->queue_chunk) tu.queue_chunk = tuning->queue_chunk;
+      if (tuning->blocks_per_cu) tu.blocks_per_cu = tuning->blocks_per_cu;
+    }
+    validate_block(tu.block_size, "adjoint");
+    if (tu.queue_chunk < 64 || tu.queue_chunk % 64 != 0 || tu.steps_per_refill == 0)
+      throw InvalidArgument("adjoint: queue_chunk must be a multiple of 64, steps_per_refill >= 1");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<DevParams> table(n_events);
+    std::vector<unsigned long long> offsets(n_events + 1);
+    offsets[0] = 0;
+    for (uint32_t e = 0; e < n_events; ++e) {
+      table[e] = to_dev(params[e]);
+      offsets[e + 1] = offsets[e] + photons[e];
+    }
+    const uint64_t total = offsets[n_events];
+    const size_t wcount = static_cast<size_t>(n_events) * (time_bins + 1u);
+    const auto ta = std::chrono::steady_clock::now();
+    bool grew = c.batch_table.ensure(table.size() * sizeof(DevParams));
+    grew |= c.batch_offsets.ensure(offsets.size() * 8);
+    grew |= c.adj_weights.ensure(wcount * 8);
+    grew |= c.adj_grad.ensure(static_cast<size_t>(n_events) * 5 * 8);
+    grew |= c.adj_queue.ensure((static_cast<size_t>(n_events) + 1) * 8);
+    if (grew) tm.alloc_ms += ms_since(ta);
+    if (grew && c.batch_exec) {  // the batch graph refers to batch_table / batch_offsets
+      HIPCHK(hipGraphExecDestroy(c.batch_exec));
+      c.batch_exec = nullptr;
+      c.batch_key.clear();
+    }
+    const auto th = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(c.batch_table.ptr, table.data(), table.size() * sizeof(DevParams),
+                          hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.batch_offsets.ptr, offsets.data(), offsets.size() * 8,
+                          hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.adj_weights.ptr, weights, wcount * 8, hipMemcpyHostToDevice,
+                          c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.h2d_ms = ms_since(th);
+
+    const int max_per_cu = std::max(1, adjoint_blocks_per_cu(tu.block_size));
+    const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
+    const uint64_t needed = (total + tu.block_size - 1) / tu.block_size;
+    WavefrontShape s{};
+    s.block = tu.block_size;
+    s.grid = static_cast<int>(std::max<uint64_t>(
+        1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
+    s.steps_per_refill = tu.steps_per_refill;
+    s.queue_chunk = tu.queue_chunk;
+
+    AdjointArgs a{};
+    a.table = c.batch_table.as<DevParams>();
+    a.offsets = c.batch_offsets.as<unsigned long long>();
+    a.qheads = c.adj_queue.as<unsigned long long>();
+    a.closed = c.adj_queue.as<unsigned long long>() + n_events;
+    a.n_events = n_events;
+    a.time_bins = time_bins;
+    a.time_scale = static_cast<float>(time_bins) / time_max;  // transport.hpp:171
+    a.weights = c.adj_weights.as<double>();
+    a.grad = c.adj_grad.as<double>();
+    launch_zero_u64(a.qheads, static_cast<uint64_t>(n_events) + 1, c.stream);
+    launch_zero_u64(reinterpret_cast<unsigned long long*>(a.grad),
+                    static_cast<uint64_t>(n_events) * 5, c.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev[0], c.stream));
+    launch_adjoint(key_of(params[0]), first_id, s, a, c.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev[1], c.stream));
+    tm.kernel_ms = c.elapsed(0, 1);
+    const auto td = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(grad, a.grad, static_cast<size_t>(n_events) * 5 * 8,
+                          hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.d2h_ms = ms_since(td);
+    for (uint32_t e = 0; e < n_events; ++e) {
+      const double inv = photons[e] ? 1.0 / static_cast<double>(photons[e]) : 0.0;
+      for (int j = 0; j < 5; ++j) grad[5 * e + j] *= inv;
+    }
     tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
   });

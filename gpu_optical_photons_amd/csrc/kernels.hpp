@@ -68,6 +68,24 @@ struct LaunchOut {
   bool records, tally_on, events;
 };
 
+// Adjoint launch: parameter sets (events) e = 0..n_events-1 with counts[e]
+// photons at ids first_id + offsets[e] ..; per-set weights w[e][k] over the
+// time bins k = 0..time_bins (overflow included); output
+// grad[e][j] += sum over detected photons of the set of
+//   w[e][bin(t)] * S_j + (j == n_scint) * time term,
+// j = 0..4 for abs_length, scat_length, reflectivity, n_det, n_scint.
+struct AdjointArgs {
+  const DevParams* table;
+  const unsigned long long* offsets;  // [n_events + 1]
+  unsigned long long* qheads;         // [n_events] per-set queue heads, zeroed
+  unsigned long long* closed;         // [1] number of sets whose queue ran out, zeroed
+  uint32_t n_events;
+  uint32_t time_bins;
+  float time_scale;                   // time_bins / time_max
+  const double* weights;              // [n_events][time_bins + 1]
+  double* grad;                       // [n_events][5], zeroed
+};
+
 struct WavefrontShape {
   int block, grid;
   uint32_t steps_per_refill, queue_chunk;
@@ -98,6 +116,12 @@ void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, 
                           int block, const LaunchOut& out, hipStream_t stream);
 
 int batch_blocks_per_cu(int block);
+
+// Adjoint (likelihood-ratio + pathwise) pass over the histories of a set of
+// parameter sets, recomputed from their Philox counters.
+void launch_adjoint(PhiloxKey key, uint64_t first_id, const WavefrontShape& shape,
+                    const AdjointArgs& a, hipStream_t stream);
+int adjoint_blocks_per_cu(int block);
 
 // Sums the per-block histograms and sums into the flat u64 tally layout.
 void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials,

@@ -243,6 +243,34 @@ struct Photon {
   uint32_t nb, ns;
 };
 
+// Likelihood-ratio scores a photon accumulates along its history (adjoint
+// kernel only): d log p(history) / d n_det and d n_scint from every Fresnel
+// decision, and the number of reflector hits (for d / d reflectivity).
+struct AdjState {
+  float s_ndet, s_nscint;
+  uint32_t refl_hits;
+};
+
+// Partial derivatives of the unpolarised Fresnel reflectance
+// R = (rs^2 + rp^2) / 2 (fresnel.hpp:30-46) with respect to n1 and n2, away
+// from total internal reflection, from the closed forms
+//   cos_t = sqrt(1 - (n1/n2)^2 (1 - cos_i^2)),
+//   rs = (n1 ci - n2 ct) / (n1 ci + n2 ct),  rp = (n2 ci - n1 ct) / (n2 ci + n1 ct).
+__device__ __forceinline__ void fresnel_dR(float n1, float n2, float ci, float ct, float rs,
+                                           float rp, float& dR_dn1, float& dR_dn2) {
+  const float s2i = __builtin_fmaxf(0.0f, 1.0f - ci * ci);
+  const float dct1 = -n1 * s2i / (n2 * n2 * ct);
+  const float dct2 = n1 * n1 * s2i / (n2 * n2 * n2 * ct);
+  const float A = n1 * ci - n2 * ct, B = n1 * ci + n2 * ct;
+  const float C = n2 * ci - n1 * ct, D = n2 * ci + n1 * ct;
+  const float drs1 = ((ci - n2 * dct1) * B - A * (ci + n2 * dct1)) / (B * B);
+  const float drs2 = ((-ct - n2 * dct2) * B - A * (ct + n2 * dct2)) / (B * B);
+  const float drp1 = ((-ct - n1 * dct1) * D - C * (ct + n1 * dct1)) / (D * D);
+  const float drp2 = ((ci - n1 * dct2) * D - C * (ci + n1 * dct2)) / (D * D);
+  dR_dn1 = rs * drs1 + rp * drp1;
+  dR_dn2 = rs * drs2 + rp * drp2;
+}
+
 // Per-step classification, exposed so kernels can count branch events.
 struct StepInfo {
   bool absorb, scatter, readout, polished, black, refl, survived, lambert_reflect;
@@ -284,9 +312,9 @@ __device__ __forceinline__ void emit(const DevParams& P, Stream& rng, Photon& ph
 // rng.refill() so that pos <= 3.
 // kSurf >= 0 fixes the surface model at compile time (a launch whose events
 // all share it); kSurf < 0 reads it from P.
-template <int kSurf = -1>
+template <int kSurf = -1, bool kAdj = false>
 __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph, uint32_t& fate,
-                                     StepInfo& info) {
+                                     StepInfo& info, AdjState* adj = nullptr) {
   const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
   // --- two exponential distances, always drawn (transport.hpp:76-77) -------
   const float u_abs = to_unit(rng.peek<0>());
@@ -358,6 +386,7 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
 
   // --- Fresnel (fresnel.hpp:30-46), once for readout and polished lanes -----
   float prob = P.reflectivity;
+  float dR1 = 0.0f, dR2 = 0.0f;  // dR/dn_scint, dR/dn2 (adjoint only)
   if (readout || polished) {
     const float n1 = P.n_scint;
     const float n2 = readout ? P.n_det : P.n_out;
@@ -368,12 +397,25 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
     const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
     const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
     const float R = 0.5f * (rs * rs + rp * rp);
-    prob = matched ? 0.0f : (sin2_t >= 1.0f ? 1.0f : R);
+    const bool tir = sin2_t >= 1.0f;
+    prob = matched ? 0.0f : (tir ? 1.0f : R);
+    // Matched interfaces (R = 0 to second order) and total internal
+    // reflection (R = 1 on an open set of angles) carry no score.
+    if (kAdj && !matched && !tir) fresnel_dR(n1, n2, cos_i, cos_t, rs, rp, dR1, dR2);
   }
   // --- the surface decision draw (u <= prob survives, transport.hpp:41) -----
   const bool decide = readout || polished || refl;
   const bool survived = decide && to_unit(rng.peek<2>()) <= prob;
   const bool lambert_reflect = refl && lambert && survived;
+  if (kAdj && (readout || polished)) {
+    // Bernoulli score: d log R (reflected) or d log (1 - R) (transmitted).
+    const float inv = survived ? 1.0f / prob : -1.0f / (1.0f - prob);
+    if (dR1 != 0.0f || dR2 != 0.0f) {
+      adj->s_nscint += dR1 * inv;
+      if (readout) adj->s_ndet += dR2 * inv;
+    }
+  }
+  if (kAdj && refl) ++adj->refl_hits;
 
   // --- new direction ----------------------------------------------------------
   if (survived && !lambert_reflect) {  // specular: flip the approached component

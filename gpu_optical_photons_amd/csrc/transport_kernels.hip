@@ -324,6 +324,155 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
 // sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
 // every gridDim.y-th block row, so enough waves are in flight to stream the
 // partials at HBM rate; each column slice adds its sum with one atomic.
+// ---------------------------------------------------------------------------
+// Adjoint kernel. Recomputes every history from its Philox counter (no tape)
+// and accumulates, per detected photon, the weight of its arrival-time bin
+// times its likelihood-ratio scores:
+//   abs_length:  path / L_a^2 - [absorbed in the bulk] / L_a
+//                (exponential free flights competing with the other events)
+//   scat_length: path / L_s^2 - n_scatter / L_s
+//   reflectivity: (reflector hits - a) / R - a / (1 - R), a = [absorbed there]
+//   n_det, n_scint: sum over Fresnel decisions of d log R or d log (1 - R)
+// plus, for n_scint, the pathwise time term: t = t_emit + path n_scint / c
+// with t_emit ~ Exp(tau), so moving t by dx shifts t_emit, and
+//   d/dx P(t in bin k) = (1/tau) (P(t in k) - [x in k]),  x = path n_scint / c,
+// which per photon is (path / c / tau) (w[bin(t)] - w[bin(x)]).
+// Scores are f32 per photon; the weighted sums are f64.
+// Work: per-set queues; a wave stays on one set until it runs out, so a lane
+// changes set rarely and flushes its f64 accumulators with few atomics.
+template <int kSurf>
+__global__ __launch_bounds__(1024) void adjoint_kernel(PhiloxKey key, uint64_t first_id,
+                                                       uint32_t steps_per_refill, uint32_t chunk,
+                                                       AdjointArgs a) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t E = a.n_events;
+  uint32_t ws = wave % E;  // the wave's current set (uniform)
+  uint32_t visited = 0;
+  uint64_t q_next = 0, q_end = 0;
+  bool w_done = false;
+
+  bool alive = false;
+  uint32_t lset = 0xffffffffu, pset = 0;  // accumulator set, photon's set
+  double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+  DevParams P{};
+  Stream rng;
+  Photon ph;
+  AdjState adj{};
+  uint32_t done = 0, fate = kMaxSteps;
+
+  auto flush = [&]() {
+    if (lset != 0xffffffffu) {
+      double* gr = a.grad + 5u * lset;
+      if (g0 != 0.0) atomicAdd(gr + 0, g0);
+      if (g1 != 0.0) atomicAdd(gr + 1, g1);
+      if (g2 != 0.0) atomicAdd(gr + 2, g2);
+      if (g3 != 0.0) atomicAdd(gr + 3, g3);
+      if (g4 != 0.0) atomicAdd(gr + 4, g4);
+    }
+    g0 = g1 = g2 = g3 = g4 = 0.0;
+  };
+
+  for (;;) {
+    const uint64_t dead = __ballot(!alive);
+    if (dead != 0 && !w_done) {
+      const uint32_t need = popc(dead);
+      const uint64_t have = q_end - q_next;
+      const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+      const uint64_t b1 = q_next;
+      const uint32_t s1 = ws;
+      q_next += t1;
+      uint64_t b2 = 0;
+      uint32_t t2 = 0;
+      while (t1 < need && !w_done) {  // uniform: claim a chunk, moving on past exhausted sets
+        const uint64_t cnt = a.offsets[ws + 1] - a.offsets[ws];
+        unsigned long long got = 0;
+        if (lane == 0) got = atomicAdd(a.qheads + ws, static_cast<unsigned long long>(chunk));
+        got = first_lane_u64(got);
+        if (got < cnt) {
+          if (got + chunk >= cnt && lane == 0) atomicAdd(a.closed, 1ull);
+          q_next = got;
+          q_end = (got + chunk < cnt) ? got + chunk : cnt;
+          t2 = (need - t1 < q_end - q_next) ? need - t1 : static_cast<uint32_t>(q_end - q_next);
+          b2 = q_next;
+          q_next += t2;
+          break;
+        }
+        ++visited;
+        unsigned long long closed = 0;
+        if (lane == 0) closed = __atomic_load_n(a.closed, __ATOMIC_RELAXED);
+        closed = first_lane_u64(closed);
+        if (visited >= E || closed >= E) {
+          w_done = true;
+        } else {
+          ws = (ws + 1u == E) ? 0u : ws + 1u;
+          q_next = q_end = 0;
+        }
+      }
+      const uint32_t r = rank_in(dead);
+      if (!alive && r < t1 + t2) {
+        pset = r < t1 ? s1 : ws;
+        const uint64_t local = r < t1 ? b1 + r : b2 + (r - t1);
+        P = a.table[pset];
+        rng.start(first_id + a.offsets[pset] + local, key, 0u);
+        emit(P, rng, ph);
+        done = 0;
+        fate = kMaxSteps;
+        adj = AdjState{0.0f, 0.0f, 0u};
+        alive = P.max_steps > 0u;
+      }
+    }
+    if (__ballot(alive) == 0) {
+      if (w_done) break;
+      continue;
+    }
+    for (uint32_t k = 0; k < steps_per_refill; ++k) {
+      if (__ballot(alive) == 0) break;
+      bool finished = false;
+      if (alive) {
+        StepInfo info{};
+        rng.refill(key);
+        const bool still = step<kSurf, true>(P, rng, ph, fate, info, &adj);
+        ++done;
+        if (!still || done >= P.max_steps) {
+          if (still) fate = kMaxSteps;
+          finished = true;
+          alive = false;
+        }
+      }
+      if (finished && fate == kDetected) {
+        const float inf = __builtin_huge_valf();
+        float s0 = 0.0f, s1f = 0.0f, s2 = 0.0f;
+        if (P.abs_length != inf) s0 = ph.path / (P.abs_length * P.abs_length);
+        if (P.scat_length != inf)
+          s1f = ph.path / (P.scat_length * P.scat_length) -
+                static_cast<float>(ph.ns) / P.scat_length;
+        const uint32_t surf = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
+        if (surf >= kSpecular) s2 = static_cast<float>(adj.refl_hits) / P.reflectivity;
+        const uint32_t bt = time_bin(ph.t, a.time_scale, a.time_bins);
+        const double* w = a.weights + static_cast<size_t>(pset) * (a.time_bins + 1u);
+        const double wt = w[bt];
+        double tterm = 0.0;
+        if (P.tau > 0.0f) {
+          const float x = ph.path * P.inv_speed;
+          const uint32_t bx = time_bin(x, a.time_scale, a.time_bins);
+          tterm = (wt - w[bx]) * static_cast<double>(ph.path / 299.792458f / P.tau);
+        }
+        if (pset != lset) {
+          flush();
+          lset = pset;
+        }
+        g0 += wt * static_cast<double>(s0);
+        g1 += wt * static_cast<double>(s1f);
+        g2 += wt * static_cast<double>(s2);
+        g3 += wt * static_cast<double>(adj.s_ndet);
+        g4 += wt * static_cast<double>(adj.s_nscint) + tterm;
+      }
+    }
+  }
+  flush();
+}
+
 __global__ void tally_reduce_kernel(uint32_t words, int blocks, const uint32_t* partials,
                                     const unsigned long long* sums, unsigned long long* tally) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,6 +618,14 @@ int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds) {
   if (records) return blocks_per_cu(wavefront_kernel<true, false, false, true>, block, lds);
   return blocks_per_cu(wavefront_kernel<false, true, false, false, kLambertian>, block, lds);
 }
+
+void launch_adjoint(PhiloxKey key, uint64_t first_id, const WavefrontShape& s,
+                    const AdjointArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL((adjoint_kernel<-1>), dim3(s.grid), dim3(s.block), 0, stream, key, first_id,
+                     s.steps_per_refill, s.queue_chunk, a);
+}
+
+int adjoint_blocks_per_cu(int block) { return blocks_per_cu(adjoint_kernel<-1>, block, 0); }
 
 int batch_blocks_per_cu(int block) {
   return blocks_per_cu(wavefront_kernel<false, false, true, false>, block, 0);

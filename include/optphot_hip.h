@@ -187,6 +187,24 @@ int optphot_run_gpu_batch(const optphot_sim_params* params, const uint64_t* phot
                           const optphot_wavefront_tuning* tuning,
                           optphot_step_counters* counters, optphot_run_timing* timing);
 
+/* Adjoint of per-set arrival-time histograms (vector-Jacobian product).
+ * Events are parameter sets, laid out as for optphot_run_gpu_batch. h_e[k]
+ * is the fraction of set e's photons detected in time bin k (k = 0 ..
+ * time_bins, the last being the overflow bin), i.e. the batch tally's time
+ * histogram divided by photons[e]. Given weights w[e][k] (f64, row-major
+ * [n_events][time_bins + 1]) the call writes
+ *   grad[e][j] = d/d theta_j  sum_k w[e][k] h_e[k]
+ * for theta = (abs_length, scat_length, reflectivity, n_det, n_scint)
+ * (grad: f64 [n_events][5]), estimated on the same photons (same seed and
+ * ids) by likelihood-ratio scores of every absorb / scatter / Fresnel /
+ * reflector decision and, for n_scint, the pathwise arrival-time term
+ * through the exponential emission time (requires tau > 0 for that term).
+ * Histories are recomputed from their Philox counters (no stored tape). */
+int optphot_run_gpu_adjoint(const optphot_sim_params* params, const uint64_t* photons,
+                            uint32_t n_events, uint64_t first_id, uint32_t time_bins,
+                            float time_max, const double* weights, double* grad,
+                            const optphot_wavefront_tuning* tuning, optphot_run_timing* timing);
+
 /* Number of compute units and the kernels' register / occupancy facts,
  * written as "key=value" lines into buf (diagnostics). */
 int optphot_device_info(char* buf, size_t len);

@@ -363,8 +363,54 @@ static pstate start_photon(const oracle_params* p, stream_t* rng) {
   return s;
 }
 
-/* transport.hpp:71-145. Returns 1 while alive, else sets *fate. */
+/* Likelihood-ratio scores of one history, in double (adjoint reference).
+   The history itself is the FP32 one above; only the scores are f64. */
+typedef struct {
+  double s_ndet, s_nscint;
+  uint32_t refl_hits;
+} adjscore;
+
+/* d log P(decision) / d n1, d n2 for a Fresnel decision at cos_i (float
+   inputs of the FP32 history), evaluated in double from
+   R = (rs^2 + rp^2)/2, cos_t = sqrt(1 - (n1/n2)^2 (1 - cos_i^2)).
+   No score for matched interfaces or total internal reflection (as decided
+   by the FP32 history). */
+static void fresnel_score(float n1f, float n2f, float cif, int reflected, double* d1, double* d2) {
+  *d1 = *d2 = 0.0;
+  if (n1f == n2f) return;
+  const float eta = n1f / n2f;
+  if (eta * eta * fmaxf(0.0f, 1.0f - cif * cif) >= 1.0f) return; /* FP32 says TIR */
+  const double n1 = n1f, n2 = n2f, ci = cif;
+  const double s2i = fmax(0.0, 1.0 - ci * ci);
+  const double s2t = (n1 / n2) * (n1 / n2) * s2i;
+  if (s2t >= 1.0) return;
+  const double ct = sqrt(1.0 - s2t);
+  const double A = n1 * ci - n2 * ct, B = n1 * ci + n2 * ct;
+  const double C = n2 * ci - n1 * ct, D = n2 * ci + n1 * ct;
+  const double rs = A / B, rp = C / D;
+  const double R = 0.5 * (rs * rs + rp * rp);
+  const double dct1 = -n1 * s2i / (n2 * n2 * ct);
+  const double dct2 = n1 * n1 * s2i / (n2 * n2 * n2 * ct);
+  const double drs1 = ((ci - n2 * dct1) * B - A * (ci + n2 * dct1)) / (B * B);
+  const double drs2 = ((-ct - n2 * dct2) * B - A * (ct + n2 * dct2)) / (B * B);
+  const double drp1 = ((-ct - n1 * dct1) * D - C * (ct + n1 * dct1)) / (D * D);
+  const double drp2 = ((ci - n1 * dct2) * D - C * (ci + n1 * dct2)) / (D * D);
+  const double dR1 = rs * drs1 + rp * drp1, dR2 = rs * drs2 + rp * drp2;
+  const double inv = reflected ? 1.0 / R : -1.0 / (1.0 - R);
+  *d1 = dR1 * inv;
+  *d2 = dR2 * inv;
+}
+
+/* transport.hpp:71-145. Returns 1 while alive, else sets *fate. `adj`
+   (optional) collects the adjoint scores of the decisions taken. */
+static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int* fate,
+                           adjscore* adj);
 static int step_photon(const oracle_params* p, stream_t* rng, pstate* s, int* fate) {
+  return step_photon_adj(p, rng, s, fate, NULL);
+}
+
+static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int* fate,
+                           adjscore* adj) {
   const float inv_speed = p->n_scint / kSpeedOfLight;
   const float s_abs = sample_exponential(p->abs_length, stream_uniform(rng));
   const float s_sca = sample_exponential(p->scat_length, stream_uniform(rng));
@@ -398,7 +444,14 @@ static int step_photon(const oracle_params* p, stream_t* rng, pstate* s, int* fa
 
   if (face == 5) { /* readout +z, :108-118 */
     const float R = fresnel_R(p->n_scint, p->n_det, cos_i);
-    if (happens(stream_uniform(rng), R)) {
+    const int refl = happens(stream_uniform(rng), R);
+    if (adj) {
+      double d1, d2;
+      fresnel_score(p->n_scint, p->n_det, cos_i, refl, &d1, &d2);
+      adj->s_nscint += d1;
+      adj->s_ndet += d2;
+    }
+    if (refl) {
       s->dir = reflect(s->dir, normal);
       return 1;
     }
@@ -407,7 +460,13 @@ static int step_photon(const oracle_params* p, stream_t* rng, pstate* s, int* fa
   }
   if (p->surface == ORACLE_SURFACE_POLISHED) { /* :120-128 */
     const float R = fresnel_R(p->n_scint, p->n_out, cos_i);
-    if (happens(stream_uniform(rng), R)) {
+    const int refl = happens(stream_uniform(rng), R);
+    if (adj) {
+      double d1, d2;
+      fresnel_score(p->n_scint, p->n_out, cos_i, refl, &d1, &d2);
+      adj->s_nscint += d1;
+    }
+    if (refl) {
       s->dir = reflect(s->dir, normal);
       return 1;
     }
@@ -418,6 +477,7 @@ static int step_photon(const oracle_params* p, stream_t* rng, pstate* s, int* fa
     *fate = ORACLE_FATE_ABSORBED_SURFACE;
     return 0;
   }
+  if (adj) ++adj->refl_hits;
   if (!happens(stream_uniform(rng), p->reflectivity)) { /* :134-137 */
     *fate = ORACLE_FATE_ABSORBED_SURFACE;
     return 0;
@@ -623,4 +683,64 @@ ORACLE_EXPORT uint32_t oracle_time_bin(float t, const oracle_tally_config* c) {
 ORACLE_EXPORT uint32_t oracle_xy_bin(float x, float y, const oracle_params* p,
                                      const oracle_tally_config* c) {
   return xy_bin(x, y, p, c);
+}
+
+/* Adjoint reference: for parameter sets e (photons[e] photons at ids
+   first_id + offset_e ..), grad[e][j] = (1/N_e) sum over detected photons of
+   w[e][bin(t)] S_j (+ the arrival-time term for n_scint), with
+   j = abs_length, scat_length, reflectivity, n_det, n_scint and the scores
+     S_abs = path / L_a^2            (detected photons were not absorbed)
+     S_sca = path / L_s^2 - n_scatter / L_s
+     S_R   = reflector hits / R
+     S_n   = sum of Fresnel decision scores
+   and, for n_scint, (w[bin(t)] - w[bin(x)]) (path / c) / tau with
+   x = path * (n_scint / c): the emission time t_e ~ Exp(tau) absorbs a shift
+   of the time of flight. Histories are the FP32 ones of transport_photon;
+   scores and sums are double. */
+ORACLE_EXPORT void oracle_adjoint(const oracle_params* params, const uint64_t* photons,
+                                  uint32_t n_events, uint64_t first_id, uint32_t time_bins,
+                                  float time_max, const double* weights, double* grad, int threads) {
+  const oracle_tally_config tc = {time_bins, time_max, 1u};
+  uint64_t offset = 0;
+  for (uint32_t e = 0; e < n_events; ++e) {
+    const oracle_params* p = params + e;
+    const double* w = weights + (size_t)e * (time_bins + 1u);
+    const int64_t n = (int64_t)photons[e];
+    double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
+    const int nt = thread_count(threads);
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(nt) reduction(+ : g0, g1, g2, g3, g4)
+    for (int64_t i = 0; i < n; ++i) {
+      stream_t rng;
+      stream_init(&rng, p->seed, first_id + offset + (uint64_t)i, 0u);
+      pstate s = start_photon(p, &rng);
+      int fate = ORACLE_FATE_MAX_STEPS;
+      adjscore adj = {0.0, 0.0, 0u};
+      for (uint32_t step = 0; step < p->max_steps; ++step) {
+        if (!step_photon_adj(p, &rng, &s, &fate, &adj)) break;
+      }
+      if (fate != ORACLE_FATE_DETECTED) continue;
+      const double path = s.path;
+      const double wt = w[time_bin(s.t, &tc)];
+      if (!is_inf(p->abs_length)) g0 += wt * (path / ((double)p->abs_length * p->abs_length));
+      if (!is_inf(p->scat_length))
+        g1 += wt * (path / ((double)p->scat_length * p->scat_length) -
+                    (double)s.ns / p->scat_length);
+      if (p->surface == ORACLE_SURFACE_SPECULAR || p->surface == ORACLE_SURFACE_LAMBERTIAN)
+        g2 += wt * ((double)adj.refl_hits / p->reflectivity);
+      g3 += wt * adj.s_ndet;
+      double tterm = 0.0;
+      if (p->tau > 0.0f) {
+        const float x = s.path * (p->n_scint / kSpeedOfLight);
+        tterm = (wt - w[time_bin(x, &tc)]) * (path / 299.792458 / p->tau);
+      }
+      g4 += wt * adj.s_nscint + tterm;
+    }
+    const double inv = n ? 1.0 / (double)n : 0.0;
+    grad[5 * e + 0] = g0 * inv;
+    grad[5 * e + 1] = g1 * inv;
+    grad[5 * e + 2] = g2 * inv;
+    grad[5 * e + 3] = g3 * inv;
+    grad[5 * e + 4] = g4 * inv;
+    offset += photons[e];
+  }
 }

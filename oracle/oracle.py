@@ -128,6 +128,21 @@ class Oracle(_Lib):
         self._fn("run_tally", None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                      ctypes.c_uint64, ctypes.c_int, _u64p])
         self._fn("u32_to_uniform", ctypes.c_float, [ctypes.c_uint32])
+        self._fn("adjoint", None, [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.c_uint64,
+                                   ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_int])
+
+    def gradient(self, params_list, photons, time_bins, time_max, weights, first_id=0,
+                 threads=0):
+        """FP64 reference of the adjoint: grad [E, 5] = d/dtheta sum_k w[e,k] h_e[k]."""
+        E = len(params_list)
+        arr = (type(params_list[0]) * E)(*params_list)
+        ph = np.ascontiguousarray(photons, np.uint64)
+        w = np.ascontiguousarray(weights, np.float64).reshape(E, time_bins + 1)
+        g = np.zeros((E, 5), np.float64)
+        self.adjoint(ctypes.addressof(arr), ph, E, first_id, time_bins, time_max,
+                     w.ctypes.data, g.ctypes.data, threads)
+        return g
 
     def records(self, params, n, first_id=0, threads=0, with_draws=False):
         r = _empty_records(n)

@@ -70,3 +70,44 @@ def mirror_box_escape_fraction(n_scint, n_det):
 
 def binomial_pull(k, n, p):
     return (k - n * p) / math.sqrt(n * p * (1.0 - p))
+
+
+def face_integral(p, face, integrand, panels=24, order=16):
+    """(1/4pi) Int over `face` of integrand(r, cos_i) dOmega for a point source,
+    dOmega = h dA / r^3 and cos_i = h / r (the angle to the face normal), by the
+    same composite Gauss-Legendre rule as face_survival_fraction."""
+    h, u1, u2, v1, v2 = face_frame(p, face)
+    gx, gw = np.polynomial.legendre.leggauss(order)
+
+    def edges(a, b):
+        cuts = [a, 0.0, b] if a < 0.0 < b else [a, b]
+        e = []
+        for k in range(len(cuts) - 1):
+            e.extend(cuts[k] + (cuts[k + 1] - cuts[k]) * np.arange(panels) / panels)
+        e.append(b)
+        return np.array(e)
+
+    eu, ev = edges(u1, u2), edges(v1, v2)
+    cu, hu = 0.5 * (eu[1:] + eu[:-1]), 0.5 * (eu[1:] - eu[:-1])
+    cv, hv = 0.5 * (ev[1:] + ev[:-1]), 0.5 * (ev[1:] - ev[:-1])
+    u = (cu[:, None] + hu[:, None] * gx[None, :]).ravel()
+    wu = (hu[:, None] * gw[None, :]).ravel()
+    v = (cv[:, None] + hv[:, None] * gx[None, :]).ravel()
+    wv = (hv[:, None] * gw[None, :]).ravel()
+    r = np.sqrt(u[:, None] ** 2 + v[None, :] ** 2 + h * h)
+    return float((wu[:, None] * wv[None, :] * h / r ** 3 * integrand(r, h / r)).sum()
+                 / (4.0 * math.pi))
+
+
+def fresnel_R(n1, n2, ci):
+    """Unpolarised Fresnel reflectance (float64, vectorised), angle form."""
+    ci = np.asarray(ci, np.float64)
+    s2t = (n1 / n2) ** 2 * (1.0 - ci * ci)
+    out = np.ones_like(ci)
+    ok = s2t < 1.0
+    ct = np.sqrt(1.0 - s2t[ok])
+    c = ci[ok]
+    rs = (n1 * c - n2 * ct) / (n1 * c + n2 * ct)
+    rp = (n2 * c - n1 * ct) / (n2 * c + n1 * ct)
+    out[ok] = 0.5 * (rs * rs + rp * rp)
+    return out
