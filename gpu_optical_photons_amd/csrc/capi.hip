@@ -630,8 +630,12 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     lo.evt.n_events = n_events;
     const PhiloxKey key = key_of(params[0]);
 
+    uint64_t max_chunks = 0;
+    for (uint32_t e = 0; e < n_events; ++e)
+      max_chunks = std::max<uint64_t>(max_chunks, (photons[e] + s.queue_chunk - 1) / s.queue_chunk);
+    const uint64_t n_slots = max_chunks * n_events;
     if (method == OPTPHOT_BATCH_WAVEFRONT) {
-      const std::string gkey = std::to_string(n_events) + "/" + std::to_string(total) + "/" +
+      const std::string gkey = std::to_string(n_events) + "/" + std::to_string(n_slots) + "/" +
                                std::to_string(first_id) + "/" + std::to_string(params[0].seed) +
                                "/" + std::to_string(words) + "/" + std::to_string(lo.evt.time_scale) +
                                "/" + std::to_string(s.block) + "/" + std::to_string(s.grid) + "/" +
@@ -644,7 +648,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
         HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
         launch_zero_u64(scratch, kScratchSums, c.stream);  // queue head + counters
         launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64, c.stream);
-        launch_batch_wavefront(key, first_id, total, s, scratch + kScratchQueue, lo, c.stream);
+        launch_batch_wavefront(key, first_id, n_slots, s, scratch + kScratchQueue, lo, c.stream);
         HIPCHK(hipStreamEndCapture(c.stream, &g));
         HIPCHK(hipGraphInstantiate(&c.batch_exec, g, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(g));

@@ -103,10 +103,12 @@ void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
                       const WavefrontShape& shape, unsigned long long* queue, const LaunchOut& out,
                       hipStream_t stream);
 
-// Batched events: one persistent launch over the concatenated photon ids of
-// all events (event e owns ids first_id + offsets[e] .. first_id + offsets[e+1]),
-// per-lane event parameters from out.evt.table, per-event tallies.
-void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_total,
+// Batched events: one persistent launch over all events (event e owns ids
+// first_id + offsets[e] .. first_id + offsets[e+1] - 1), per-lane event
+// parameters from out.evt.table, per-event tallies. The queue has n_slots =
+// n_events * max_e ceil(photons_e / queue_chunk) slots; slot c is chunk
+// c / n_events of event c mod n_events (empty slots are skipped).
+void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
                             const WavefrontShape& shape, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream);
 

@@ -251,24 +251,37 @@ struct AdjState {
   uint32_t refl_hits;
 };
 
-// Partial derivatives of the unpolarised Fresnel reflectance
-// R = (rs^2 + rp^2) / 2 (fresnel.hpp:30-46) with respect to n1 and n2, away
-// from total internal reflection, from the closed forms
+// Score of one Fresnel decision: d log P(decision) / d n1 and / d n2 with
+// P = R (reflected) or 1 - R (transmitted), R = (rs^2 + rp^2) / 2 and
 //   cos_t = sqrt(1 - (n1/n2)^2 (1 - cos_i^2)),
 //   rs = (n1 ci - n2 ct) / (n1 ci + n2 ct),  rp = (n2 ci - n1 ct) / (n2 ci + n1 ct).
-__device__ __forceinline__ void fresnel_dR(float n1, float n2, float ci, float ct, float rs,
-                                           float rp, float& dR_dn1, float& dR_dn2) {
-  const float s2i = __builtin_fmaxf(0.0f, 1.0f - ci * ci);
-  const float dct1 = -n1 * s2i / (n2 * n2 * ct);
-  const float dct2 = n1 * n1 * s2i / (n2 * n2 * n2 * ct);
-  const float A = n1 * ci - n2 * ct, B = n1 * ci + n2 * ct;
-  const float C = n2 * ci - n1 * ct, D = n2 * ci + n1 * ct;
-  const float drs1 = ((ci - n2 * dct1) * B - A * (ci + n2 * dct1)) / (B * B);
-  const float drs2 = ((-ct - n2 * dct2) * B - A * (ct + n2 * dct2)) / (B * B);
-  const float drp1 = ((-ct - n1 * dct1) * D - C * (ct + n1 * dct1)) / (D * D);
-  const float drp2 = ((ci - n1 * dct2) * D - C * (ci + n1 * dct2)) / (D * D);
-  dR_dn1 = rs * drs1 + rp * drp1;
-  dR_dn2 = rs * drs2 + rp * drp2;
+// Evaluated in f64 from the f32 inputs of the history: close to total
+// internal reflection cos_t is tiny, 1 - R and d cos_t / dn ~ 1 / cos_t lose
+// most of their bits in f32, and those are the photons with the largest
+// scores. 1 - R uses the cancellation-free form
+//   1 - R = 2 n1 n2 ci ct (1 / (n1 ci + n2 ct)^2 + 1 / (n2 ci + n1 ct)^2).
+__device__ __forceinline__ void fresnel_score(float n1f, float n2f, float cif, bool reflected,
+                                              float& d1, float& d2) {
+  const double n1 = n1f, n2 = n2f, ci = cif;
+  const double s2i = 1.0 - ci * ci > 0.0 ? 1.0 - ci * ci : 0.0;
+  const double s2t = (n1 / n2) * (n1 / n2) * s2i;
+  d1 = d2 = 0.0f;
+  if (s2t >= 1.0) return;
+  const double ct = __builtin_sqrt(1.0 - s2t);
+  const double A = n1 * ci - n2 * ct, B = n1 * ci + n2 * ct;
+  const double C = n2 * ci - n1 * ct, D = n2 * ci + n1 * ct;
+  const double rs = A / B, rp = C / D;
+  const double dct1 = -n1 * s2i / (n2 * n2 * ct);
+  const double dct2 = n1 * n1 * s2i / (n2 * n2 * n2 * ct);
+  const double drs1 = ((ci - n2 * dct1) * B - A * (ci + n2 * dct1)) / (B * B);
+  const double drs2 = ((-ct - n2 * dct2) * B - A * (ct + n2 * dct2)) / (B * B);
+  const double drp1 = ((-ct - n1 * dct1) * D - C * (ct + n1 * dct1)) / (D * D);
+  const double drp2 = ((ci - n1 * dct2) * D - C * (ci + n1 * dct2)) / (D * D);
+  const double dR1 = rs * drs1 + rp * drp1, dR2 = rs * drs2 + rp * drp2;
+  const double inv = reflected ? 2.0 / (rs * rs + rp * rp)
+                               : -1.0 / (2.0 * n1 * n2 * ci * ct * (1.0 / (B * B) + 1.0 / (D * D)));
+  d1 = static_cast<float>(dR1 * inv);
+  d2 = static_cast<float>(dR2 * inv);
 }
 
 // Per-step classification, exposed so kernels can count branch events.
@@ -386,7 +399,7 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
 
   // --- Fresnel (fresnel.hpp:30-46), once for readout and polished lanes -----
   float prob = P.reflectivity;
-  float dR1 = 0.0f, dR2 = 0.0f;  // dR/dn_scint, dR/dn2 (adjoint only)
+  bool scored = false;  // a Fresnel decision with a score (adjoint only)
   if (readout || polished) {
     const float n1 = P.n_scint;
     const float n2 = readout ? P.n_det : P.n_out;
@@ -401,19 +414,18 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
     prob = matched ? 0.0f : (tir ? 1.0f : R);
     // Matched interfaces (R = 0 to second order) and total internal
     // reflection (R = 1 on an open set of angles) carry no score.
-    if (kAdj && !matched && !tir) fresnel_dR(n1, n2, cos_i, cos_t, rs, rp, dR1, dR2);
+    if (kAdj) scored = !matched && !tir;
   }
   // --- the surface decision draw (u <= prob survives, transport.hpp:41) -----
   const bool decide = readout || polished || refl;
   const bool survived = decide && to_unit(rng.peek<2>()) <= prob;
   const bool lambert_reflect = refl && lambert && survived;
-  if (kAdj && (readout || polished)) {
+  if (kAdj && scored) {
     // Bernoulli score: d log R (reflected) or d log (1 - R) (transmitted).
-    const float inv = survived ? 1.0f / prob : -1.0f / (1.0f - prob);
-    if (dR1 != 0.0f || dR2 != 0.0f) {
-      adj->s_nscint += dR1 * inv;
-      if (readout) adj->s_ndet += dR2 * inv;
-    }
+    float d1, d2;
+    fresnel_score(P.n_scint, readout ? P.n_det : P.n_out, cos_i, survived, d1, d2);
+    adj->s_nscint += d1;
+    if (readout) adj->s_ndet += d2;
   }
   if (kAdj && refl) ++adj->refl_hits;
 

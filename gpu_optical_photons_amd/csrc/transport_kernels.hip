@@ -201,32 +201,14 @@ __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey ke
 }
 
 // ---------------------------------------------------------------------------
-// Event of batch photon `idx`: the largest e with offsets[e] <= idx (events
-// with no photons have equal offsets and are never chosen).
-__device__ __forceinline__ uint32_t find_event(const unsigned long long* offsets, uint32_t n_events,
-                                               uint64_t idx) {
-  uint32_t lo = 0, hi = n_events;
-  while (hi - lo > 1u) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (offsets[mid] <= idx)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-// Persistent wavefront kernel (see the file comment). With kBatch the lanes
-// take photons of many events: each lane looks up its photon's event when it
-// is refilled and carries that event's parameters in registers.
-template <bool kRec, bool kTally, bool kBatch, bool kCount, int kSurf = -1>
+// Persistent wavefront kernel (see the file comment).
+template <bool kRec, bool kTally, bool kCount, int kSurf = -1>
 __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey key,
                                                          uint64_t first_id, uint64_t n,
                                                          uint32_t steps_per_refill, uint32_t chunk,
                                                          unsigned long long* queue,
                                                          LaunchOut out) {
-  DevParams P = P0;
-  uint32_t ev = 0;
+  const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
   if (kTally) {
     zero_lds(lds, out.tally.words);
@@ -274,10 +256,6 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
       bool fresh = false;
       if (!alive && r < t1 + t2) {
         idx = r < t1 ? b1 + r : b2 + (r - t1);
-        if (kBatch) {
-          ev = find_event(out.evt.offsets, out.evt.n_events, idx);
-          P = out.evt.table[ev];
-        }
         rng.start(first_id + idx, key, 0u);
         emit(P, rng, ph);
         done = 0;
@@ -287,7 +265,7 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
       }
       wc.add(kCntEmitted, t1 + t2);
       if (fresh && !alive) {  // max_steps == 0: finished at emission
-        finish<kRec, kTally, kBatch>(P, out, lds, idx, ph, fate, snb, sns, ev);
+        finish<kRec, kTally>(P, out, lds, idx, ph, fate, snb, sns);
       }
     }
     if (__ballot(alive) == 0) {
@@ -313,7 +291,7 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
       }
       wc.add_step(act, info);
       if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
-      if (finished) finish<kRec, kTally, kBatch>(P, out, lds, idx, ph, fate, snb, sns, ev);
+      if (finished) finish<kRec, kTally>(P, out, lds, idx, ph, fate, snb, sns);
     }
   }
   if (kTally) __syncthreads();
@@ -324,6 +302,109 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
 // sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
 // every gridDim.y-th block row, so enough waves are in flight to stream the
 // partials at HBM rate; each column slice adds its sum with one atomic.
+// ---------------------------------------------------------------------------
+// Batched events. Lanes take photons of many events and carry their event's
+// parameters in registers (loaded when the lane is refilled). The queue hands
+// out slots c = 0, 1, ...; slot c is chunk c / E of event c mod E, so the
+// waves in flight at any moment work on many different events and the
+// per-event tally atomics spread over many rows instead of queueing on one.
+template <bool kCount>
+__global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t first_id,
+                                                     uint32_t steps_per_refill, uint32_t chunk,
+                                                     uint64_t n_slots, unsigned long long* queue,
+                                                     LaunchOut out) {
+  const uint32_t lane = lane_id();
+  const uint32_t E = out.evt.n_events;
+  const unsigned long long* off = out.evt.offsets;
+  WaveCounters<kCount> wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
+
+  uint32_t we = 0;                 // event of the wave's claimed range (uniform)
+  uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
+  bool q_done = false;
+  bool alive = false;
+  uint32_t pset = 0;
+  uint64_t idx = 0;
+  uint32_t done = 0, fate = kMaxSteps;
+  DevParams P{};
+  Stream rng;
+  Photon ph;
+
+  for (;;) {
+    const uint64_t dead = __ballot(!alive);
+    if (dead != 0 && !q_done) {
+      const uint32_t need = popc(dead);
+      const uint64_t have = q_end - q_next;
+      const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+      const uint64_t b1 = q_next;
+      const uint32_t s1 = we;
+      q_next += t1;
+      uint64_t b2 = 0;
+      uint32_t t2 = 0;
+      while (t1 < need && !q_done) {  // uniform: claim the next non-empty slot
+        unsigned long long c = 0;
+        if (lane == 0) c = atomicAdd(queue, 1ull);
+        c = first_lane_u64(c);
+        if (c >= n_slots) {
+          q_done = true;
+          break;
+        }
+        const uint32_t e = static_cast<uint32_t>(c % E);
+        const uint64_t start = (c / E) * chunk;
+        const uint64_t ne = off[e + 1] - off[e];
+        if (start >= ne) continue;
+        we = e;
+        q_next = start;
+        q_end = (start + chunk < ne) ? start + chunk : ne;
+        t2 = (need - t1 < q_end - q_next) ? need - t1 : static_cast<uint32_t>(q_end - q_next);
+        b2 = q_next;
+        q_next += t2;
+        break;  // one chunk per refill
+      }
+      const uint32_t r = rank_in(dead);
+      bool fresh = false;
+      if (!alive && r < t1 + t2) {
+        pset = r < t1 ? s1 : we;
+        idx = r < t1 ? b1 + r : b2 + (r - t1);
+        P = out.evt.table[pset];
+        rng.start(first_id + off[pset] + idx, key, 0u);
+        emit(P, rng, ph);
+        done = 0;
+        fate = kMaxSteps;
+        alive = P.max_steps > 0u;
+        fresh = true;
+      }
+      wc.add(kCntEmitted, t1 + t2);
+      if (fresh && !alive) finish<false, false, true>(P, out, nullptr, idx, ph, fate, snb, sns, pset);
+    }
+    if (__ballot(alive) == 0) {
+      if (q_done) break;
+      continue;
+    }
+    for (uint32_t k = 0; k < steps_per_refill; ++k) {
+      const uint64_t act = __ballot(alive);
+      if (act == 0) break;
+      StepInfo info{};
+      bool finished = false;
+      if (alive) {
+        rng.refill(key);
+        const bool still = step(P, rng, ph, fate, info);
+        ++done;
+        if (!still || done >= P.max_steps) {
+          if (still) fate = kMaxSteps;
+          finished = true;
+          alive = false;
+        }
+      }
+      wc.add_step(act, info);
+      if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+      if (finished) finish<false, false, true>(P, out, nullptr, idx, ph, fate, snb, sns, pset);
+    }
+  }
+  wc.flush(out.counters);
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint kernel. Recomputes every history from its Philox counter (no tape)
 // and accumulates, per detected photon, the weight of its arrival-time bin
@@ -502,12 +583,12 @@ void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
                      first_id, n, out);
 }
 
-template <bool R, bool T, bool B, bool C, int S = -1>
+template <bool R, bool T, bool C, int S = -1>
 void launch_wavefront_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                         const WavefrontShape& s, unsigned long long* queue, const LaunchOut& out,
                         hipStream_t stream) {
   const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
-  hipLaunchKernelGGL((wavefront_kernel<R, T, B, C, S>), dim3(s.grid), dim3(s.block), lds, stream,
+  hipLaunchKernelGGL((wavefront_kernel<R, T, C, S>), dim3(s.grid), dim3(s.block), lds, stream,
                      P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
 }
 
@@ -518,18 +599,18 @@ void launch_wavefront_tally(const DevParams& P, PhiloxKey key, uint64_t first_id
                             const LaunchOut& out, hipStream_t stream) {
   switch (P.surface) {
     case kPolished:
-      launch_wavefront_t<false, true, false, C, kPolished>(P, key, first_id, n, s, queue, out,
+      launch_wavefront_t<false, true, C, kPolished>(P, key, first_id, n, s, queue, out,
                                                            stream);
       break;
     case kBlack:
-      launch_wavefront_t<false, true, false, C, kBlack>(P, key, first_id, n, s, queue, out, stream);
+      launch_wavefront_t<false, true, C, kBlack>(P, key, first_id, n, s, queue, out, stream);
       break;
     case kSpecular:
-      launch_wavefront_t<false, true, false, C, kSpecular>(P, key, first_id, n, s, queue, out,
+      launch_wavefront_t<false, true, C, kSpecular>(P, key, first_id, n, s, queue, out,
                                                            stream);
       break;
     default:
-      launch_wavefront_t<false, true, false, C, kLambertian>(P, key, first_id, n, s, queue, out,
+      launch_wavefront_t<false, true, C, kLambertian>(P, key, first_id, n, s, queue, out,
                                                              stream);
       break;
   }
@@ -564,7 +645,7 @@ void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
                       hipStream_t stream) {
   const bool count = out.counters != nullptr;
   if (out.records && !out.tally_on)
-    launch_wavefront_t<true, false, false, true>(P, key, first_id, n, shape, queue, out, stream);
+    launch_wavefront_t<true, false, true>(P, key, first_id, n, shape, queue, out, stream);
   else if (out.tally_on && !out.records && count)
     launch_wavefront_tally<true>(P, key, first_id, n, shape, queue, out, stream);
   else if (out.tally_on && !out.records)
@@ -573,15 +654,15 @@ void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
     throw std::invalid_argument("launch_wavefront: unsupported output combination");
 }
 
-void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_total,
-                            const WavefrontShape& shape, unsigned long long* queue,
+void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
+                            const WavefrontShape& s, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream) {
   if (out.counters)
-    launch_wavefront_t<false, false, true, true>(DevParams{}, key, first_id, n_total, shape, queue,
-                                                 out, stream);
+    hipLaunchKernelGGL((batch_kernel<true>), dim3(s.grid), dim3(s.block), 0, stream, key,
+                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
   else
-    launch_wavefront_t<false, false, true, false>(DevParams{}, key, first_id, n_total, shape,
-                                                  queue, out, stream);
+    hipLaunchKernelGGL((batch_kernel<false>), dim3(s.grid), dim3(s.block), 0, stream, key,
+                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
 }
 
 void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
@@ -615,8 +696,8 @@ int history_blocks_per_cu(bool tally, int block, size_t lds) {
 }
 
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds) {
-  if (records) return blocks_per_cu(wavefront_kernel<true, false, false, true>, block, lds);
-  return blocks_per_cu(wavefront_kernel<false, true, false, false, kLambertian>, block, lds);
+  if (records) return blocks_per_cu(wavefront_kernel<true, false, true>, block, lds);
+  return blocks_per_cu(wavefront_kernel<false, true, false, kLambertian>, block, lds);
 }
 
 void launch_adjoint(PhiloxKey key, uint64_t first_id, const WavefrontShape& s,
@@ -628,7 +709,7 @@ void launch_adjoint(PhiloxKey key, uint64_t first_id, const WavefrontShape& s,
 int adjoint_blocks_per_cu(int block) { return blocks_per_cu(adjoint_kernel<-1>, block, 0); }
 
 int batch_blocks_per_cu(int block) {
-  return blocks_per_cu(wavefront_kernel<false, false, true, false>, block, 0);
+  return blocks_per_cu(batch_kernel<false>, block, 0);
 }
 
 }  // namespace ophip

@@ -396,7 +396,9 @@ static void fresnel_score(float n1f, float n2f, float cif, int reflected, double
   const double drp1 = ((-ct - n1 * dct1) * D - C * (ct + n1 * dct1)) / (D * D);
   const double drp2 = ((ci - n1 * dct2) * D - C * (ci + n1 * dct2)) / (D * D);
   const double dR1 = rs * drs1 + rp * drp1, dR2 = rs * drs2 + rp * drp2;
-  const double inv = reflected ? 1.0 / R : -1.0 / (1.0 - R);
+  /* 1 - R without cancellation: 2 n1 n2 ci ct (1/B^2 + 1/D^2) */
+  const double T = 2.0 * n1 * n2 * ci * ct * (1.0 / (B * B) + 1.0 / (D * D));
+  const double inv = reflected ? 1.0 / R : -1.0 / T;
   *d1 = dR1 * inv;
   *d2 = dR2 * inv;
 }
