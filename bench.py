@@ -54,13 +54,106 @@ def cpu_baseline(p, c):
     t0 = time.perf_counter()
     lib.tally(p, probe, c.time_bins, c.time_max, c.xy_bins, threads=0)
     dt = time.perf_counter() - t0
-    n = int(min(N_PHOTONS, max(probe, 10.0 / max(dt, 1e-6) * probe)))
+    n = int(min(2_000_000_000, max(probe, 10.0 / max(dt, 1e-6) * probe)))
     t0 = time.perf_counter()
     lib.tally(p, n, c.time_bins, c.time_max, c.xy_bins, threads=0)
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "photons/s", "cores": cores, "kind": kind,
             "sample": f"photon ids 0..{n - 1} of the same workload (tally mode), "
                       f"{dt:.1f} s wall on {cores} OpenMP threads"}
+
+
+def bench_plain_small():
+    """Plain method, small: 50 mm cube, polished faces to n_out 1.0, readout
+    n_det 1.5, abs 1000 mm, no Rayleigh, point source at the centre, tau 4 ns,
+    1e6 photons, seed 1; records. History kernel (one lane per photon) and
+    the wavefront kernel (records identical bit for bit)."""
+    import numpy as np
+
+    from gpu_optical_photons_amd import backend
+    from gpu_optical_photons_amd.params import SimParams
+    p = SimParams(n_det=1.5, abs_length=1000.0, seed=1)
+    n = 1_000_000
+    backend.run_gpu_records(p, n)
+    rec, tm = backend.run_gpu_records(p, n)
+    ev, tme, st = backend.run_gpu_records_event(p, n, 2)
+    same = all(np.array_equal(getattr(rec, f).view(np.uint8), getattr(ev, f).view(np.uint8))
+               for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"))
+    steps = int(rec.steps().sum())
+    return {"photons": n, "history_kernel_ms": tm.kernel_ms,
+            "history_photons_per_s": n / (tm.kernel_ms * 1e-3),
+            "history_photon_steps_per_s": steps / (tm.kernel_ms * 1e-3),
+            "records_d2h_ms": tm.d2h_ms,
+            "wavefront_kernel_ms": tme.kernel_ms,
+            "wavefront_photons_per_s": n / (tme.kernel_ms * 1e-3),
+            "wavefront_simt_efficiency": st.simt_efficiency(),
+            "records_identical": bool(same), "photon_steps": steps}
+
+
+def bench_batched():
+    """Batched events: 4096 events (box ~U[10,100] mm per axis, surface model
+    uniform over polished / black / specular / Lambertian, n_det ~U[1.4,1.6],
+    abs ~U[300,3000] mm, scat inf or ~U[100,1000] mm, point source uniform,
+    N ~ Poisson(1e4 E), E ~U[0.1,3] MeV; generator seed 2024, run seed 1),
+    per-event detected count + 128-bin time histogram 0-50 ns. One persistent
+    graph-captured launch vs one history-kernel launch per event."""
+    import numpy as np
+
+    from gpu_optical_photons_amd.batch import PLAIN, WAVEFRONT, make_events, run_gpu_batch
+    b = make_events(4096, seed=2024)
+    run_gpu_batch(b, 128, 50.0, method=WAVEFRONT)  # capture
+    w, tw, _ = run_gpu_batch(b, 128, 50.0, method=WAVEFRONT)
+    pl, tp, _ = run_gpu_batch(b, 128, 50.0, method=PLAIN)
+    return {"events": b.n_events, "photons": b.total, "wavefront_ms": tw.kernel_ms,
+            "plain_ms": tp.kernel_ms, "wavefront_photons_per_s": b.total / (tw.kernel_ms * 1e-3),
+            "plain_photons_per_s": b.total / (tp.kernel_ms * 1e-3),
+            "identical": bool(np.array_equal(w, pl))}
+
+
+def bench_differentiation():
+    """Single-geometry differentiation: plain-small box with scat 300 mm,
+    1e7 photons, loss = chi-square against a target at (abs 1500, scat 400,
+    R 0.95, n_det 1.45, n_scint 1.60), seed 777. Forward (tally) + adjoint
+    (likelihood-ratio + pathwise) timings, and the gradient against central
+    finite differences (relative step 1e-3, common random numbers) of the
+    linearised loss on 4e7 photons in four disjoint id ranges."""
+    import math
+
+    import numpy as np
+
+    from gpu_optical_photons_amd import calibrate as cal
+    n = 10_000_000
+    tpl, T = cal.c4_problem(n)
+    theta = cal.C4_NOMINAL.copy()
+    cal.loss_and_grad(tpl, theta, T, n, n)  # warm
+    loss, g, f_ms, a_ms, dl_dh = cal.loss_and_grad(tpl, theta, T, n, n)
+    nc = 400_000_000
+    lr, lr_se = cal.lr_gradient(tpl, theta, T, nc, n, dl_dh=dl_dh)
+    fd, fd_se = cal.fd_gradient(tpl, theta, T, nc, n, rel_step=1e-3, dl_dh=dl_dh)
+    names = ("abs_length", "scat_length", "reflectivity", "n_det", "n_scint")
+    check = {}
+    for j, nm in enumerate(names):
+        se = math.hypot(lr_se[j], fd_se[j])
+        check[nm] = {"adjoint": lr[j], "adjoint_se": lr_se[j], "fd": fd[j], "fd_se": fd_se[j],
+                     "rel_diff": (lr[j] - fd[j]) / abs(fd[j]) if fd[j] else 0.0,
+                     "pull": (lr[j] - fd[j]) / se if se else 0.0}
+    return {"photons": n, "loss": loss, "grad": dict(zip(names, g.tolist())),
+            "forward_ms": f_ms, "adjoint_ms": a_ms,
+            "fwd_adj_over_fwd": (f_ms + a_ms) / f_ms,
+            "fwd_adj_photons_per_s": n / ((f_ms + a_ms) * 1e-3),
+            "gradient_check_photons": nc, "gradient_vs_fd": check}
+
+
+def bench_calibration():
+    """Batched differentiable calibration: 256 parameter sets x 1e6 photons,
+    long-histories geometry, (abs_length, R, n_det) fitted by 50 Adam
+    iterations to targets from hidden parameters under a held-out seed."""
+    from gpu_optical_photons_amd import calibrate as cal
+    r = cal.calibrate(n_sets=256, photons=1_000_000, iters=50)
+    return {k: r[k] for k in ("n_sets", "photons_per_set", "iterations", "median_rel_error",
+                              "p90_rel_error", "forward_ms", "adjoint_ms", "wall_s",
+                              "forward_photons_per_s", "fwd_adj_photons_per_s")} | {
+        "final_median_loss": r["losses"][-1], "first_median_loss": r["losses"][0]}
 
 
 def load_traffic():
@@ -80,6 +173,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--photons", type=int, default=N_PHOTONS)
+    ap.add_argument("--only-main", action="store_true",
+                    help="only the long-histories line (no other workloads)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (torch), the known-size "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -192,6 +287,13 @@ def main():
     if tr:
         line["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
         line["roofline"]["traffic_source"] = tr.get("source")
+    if not args.only_main:
+        line["plain_small"] = bench_plain_small()
+        line["batched_events"] = bench_batched()
+        line["batched_vs_plain"] = (line["batched_events"]["plain_ms"]
+                                    / line["batched_events"]["wavefront_ms"])
+        line["differentiation"] = bench_differentiation()
+        line["calibration"] = bench_calibration()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, c)
     print(json.dumps(line), flush=True)

@@ -594,7 +594,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
       }
     }
     const WavefrontShape s = [&] {
-      const int max_per_cu = std::max(1, batch_blocks_per_cu(tu.block_size));
+      const int max_per_cu =
+          std::max(1, batch_blocks_per_cu(tu.block_size, batch_lds_bytes(n_events, words)));
       const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
       const uint64_t needed = (total + tu.block_size - 1) / tu.block_size;
       WavefrontShape w{};

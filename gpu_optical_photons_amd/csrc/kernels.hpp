@@ -117,7 +117,11 @@ void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
 void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                           int block, const LaunchOut& out, hipStream_t stream);
 
-int batch_blocks_per_cu(int block);
+// Per-block LDS copies of the event tallies are used when they take at most
+// kBatchLdsMax bytes; batch_lds_bytes() is that size, or 0 for global atomics.
+constexpr size_t kBatchLdsMax = 32768;
+size_t batch_lds_bytes(uint32_t n_events, uint32_t words);
+int batch_blocks_per_cu(int block, size_t lds);
 
 // Adjoint (likelihood-ratio + pathwise) pass over the histories of a set of
 // parameter sets, recomputed from their Philox counters.

@@ -308,11 +308,24 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
 // out slots c = 0, 1, ...; slot c is chunk c / E of event c mod E, so the
 // waves in flight at any moment work on many different events and the
 // per-event tally atomics spread over many rows instead of queueing on one.
-template <bool kCount>
+//
+// With kLds (few events: n_events * words u32 fit in LDS) each block keeps
+// its own copy of all event tallies in LDS and adds it to the global rows
+// once at the end; otherwise finished photons add to the global rows
+// directly, spread over many events by the interleaved queue.
+template <bool kCount, bool kLds>
 __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t first_id,
                                                      uint32_t steps_per_refill, uint32_t chunk,
                                                      uint64_t n_slots, unsigned long long* queue,
                                                      LaunchOut out) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lds_words = kLds ? out.evt.n_events * out.evt.words : 0u;
+  if (kLds) {
+    zero_lds(lds, lds_words);
+    __syncthreads();
+  }
+  LaunchOut loc = out;
+  if (kLds) loc.evt.counts = lds;  // finish() adds into the block's copy
   const uint32_t lane = lane_id();
   const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
@@ -376,7 +389,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
         fresh = true;
       }
       wc.add(kCntEmitted, t1 + t2);
-      if (fresh && !alive) finish<false, false, true>(P, out, nullptr, idx, ph, fate, snb, sns, pset);
+      if (fresh && !alive) finish<false, false, true>(P, loc, nullptr, idx, ph, fate, snb, sns, pset);
     }
     if (__ballot(alive) == 0) {
       if (q_done) break;
@@ -399,10 +412,15 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
       }
       wc.add_step(act, info);
       if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
-      if (finished) finish<false, false, true>(P, out, nullptr, idx, ph, fate, snb, sns, pset);
+      if (finished) finish<false, false, true>(P, loc, nullptr, idx, ph, fate, snb, sns, pset);
     }
   }
   wc.flush(out.counters);
+  if (kLds) {
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < lds_words; w += blockDim.x)
+      if (lds[w]) atomicAdd(out.evt.counts + w, lds[w]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -657,11 +675,18 @@ void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
 void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
                             const WavefrontShape& s, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream) {
-  if (out.counters)
-    hipLaunchKernelGGL((batch_kernel<true>), dim3(s.grid), dim3(s.block), 0, stream, key,
+  const size_t lds = batch_lds_bytes(out.evt.n_events, out.evt.words);
+  if (lds && out.counters)
+    hipLaunchKernelGGL((batch_kernel<true, true>), dim3(s.grid), dim3(s.block), lds, stream, key,
+                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
+  else if (lds)
+    hipLaunchKernelGGL((batch_kernel<false, true>), dim3(s.grid), dim3(s.block), lds, stream,
+                       key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
+  else if (out.counters)
+    hipLaunchKernelGGL((batch_kernel<true, false>), dim3(s.grid), dim3(s.block), 0, stream, key,
                        first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
   else
-    hipLaunchKernelGGL((batch_kernel<false>), dim3(s.grid), dim3(s.block), 0, stream, key,
+    hipLaunchKernelGGL((batch_kernel<false, false>), dim3(s.grid), dim3(s.block), 0, stream, key,
                        first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
 }
 
@@ -708,8 +733,14 @@ void launch_adjoint(PhiloxKey key, uint64_t first_id, const WavefrontShape& s,
 
 int adjoint_blocks_per_cu(int block) { return blocks_per_cu(adjoint_kernel<-1>, block, 0); }
 
-int batch_blocks_per_cu(int block) {
-  return blocks_per_cu(batch_kernel<false>, block, 0);
+size_t batch_lds_bytes(uint32_t n_events, uint32_t words) {
+  const size_t b = static_cast<size_t>(n_events) * words * sizeof(uint32_t);
+  return b <= kBatchLdsMax ? b : 0;
+}
+
+int batch_blocks_per_cu(int block, size_t lds) {
+  return lds ? blocks_per_cu(batch_kernel<false, true>, block, lds)
+             : blocks_per_cu(batch_kernel<false, false>, block, 0);
 }
 
 }  // namespace ophip
