@@ -763,7 +763,10 @@ __attribute__((visibility("default"))) int optphot_run_gpu_adjoint(
                     static_cast<uint64_t>(n_events) * 5, c.stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev[0], c.stream));
-    launch_adjoint(key_of(params[0]), first_id, s, a, c.stream);
+    int surface = static_cast<int>(params[0].surface);
+    for (uint32_t e = 1; e < n_events; ++e)
+      if (static_cast<int>(params[e].surface) != surface) surface = -1;
+    launch_adjoint(table[0], surface, key_of(params[0]), first_id, s, a, c.stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev[1], c.stream));
     tm.kernel_ms = c.elapsed(0, 1);

@@ -125,8 +125,10 @@ int batch_blocks_per_cu(int block, size_t lds);
 
 // Adjoint (likelihood-ratio + pathwise) pass over the histories of a set of
 // parameter sets, recomputed from their Philox counters.
-void launch_adjoint(PhiloxKey key, uint64_t first_id, const WavefrontShape& shape,
-                    const AdjointArgs& a, hipStream_t stream);
+// surface: the model shared by every set (0..3), or -1 if they differ. With
+// a single set its parameters P0 are passed as kernel arguments.
+void launch_adjoint(const DevParams& P0, int surface, PhiloxKey key, uint64_t first_id,
+                    const WavefrontShape& shape, const AdjointArgs& a, hipStream_t stream);
 int adjoint_blocks_per_cu(int block);
 
 // Sums the per-block histograms and sums into the flat u64 tally layout.

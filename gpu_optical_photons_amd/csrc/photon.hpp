@@ -260,26 +260,47 @@ struct AdjState {
 // most of their bits in f32, and those are the photons with the largest
 // scores. 1 - R uses the cancellation-free form
 //   1 - R = 2 n1 n2 ci ct (1 / (n1 ci + n2 ct)^2 + 1 / (n2 ci + n1 ct)^2).
+// f64 reciprocal and inverse square root for the scores: an f32 hardware
+// estimate refined by two Newton steps (each squares the relative error:
+// ~2^-22 -> 2^-44 -> below f64 rounding). Arguments are positive, normal and
+// within f32 range here.
+__device__ __forceinline__ double rcp64(double x) {
+  double r = static_cast<double>(__builtin_amdgcn_rcpf(static_cast<float>(x)));
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ double rsqrt64(double x) {
+  double r = static_cast<double>(__builtin_amdgcn_rsqf(static_cast<float>(x)));
+  r = r * __builtin_fma(-0.5 * x, r * r, 1.5);
+  r = r * __builtin_fma(-0.5 * x, r * r, 1.5);
+  return r;
+}
+
 __device__ __forceinline__ void fresnel_score(float n1f, float n2f, float cif, bool reflected,
                                               float& d1, float& d2) {
   const double n1 = n1f, n2 = n2f, ci = cif;
   const double s2i = 1.0 - ci * ci > 0.0 ? 1.0 - ci * ci : 0.0;
-  const double s2t = (n1 / n2) * (n1 / n2) * s2i;
+  const double rn2 = rcp64(n2);
+  const double eta = n1 * rn2;
+  const double v = 1.0 - eta * eta * s2i;  // cos_t^2
   d1 = d2 = 0.0f;
-  if (s2t >= 1.0) return;
-  const double ct = __builtin_sqrt(1.0 - s2t);
+  if (!(v > 0.0)) return;
+  const double rct = rsqrt64(v);  // 1 / cos_t
+  const double ct = v * rct;
   const double A = n1 * ci - n2 * ct, B = n1 * ci + n2 * ct;
   const double C = n2 * ci - n1 * ct, D = n2 * ci + n1 * ct;
-  const double rs = A / B, rp = C / D;
-  const double dct1 = -n1 * s2i / (n2 * n2 * ct);
-  const double dct2 = n1 * n1 * s2i / (n2 * n2 * n2 * ct);
-  const double drs1 = ((ci - n2 * dct1) * B - A * (ci + n2 * dct1)) / (B * B);
-  const double drs2 = ((-ct - n2 * dct2) * B - A * (ct + n2 * dct2)) / (B * B);
-  const double drp1 = ((-ct - n1 * dct1) * D - C * (ct + n1 * dct1)) / (D * D);
-  const double drp2 = ((ci - n1 * dct2) * D - C * (ci + n1 * dct2)) / (D * D);
+  const double rB = rcp64(B), rD = rcp64(D);
+  const double rs = A * rB, rp = C * rD;
+  const double dct1 = -n1 * s2i * rn2 * rn2 * rct;
+  const double dct2 = n1 * n1 * s2i * rn2 * rn2 * rn2 * rct;
+  const double drs1 = ((ci - n2 * dct1) * B - A * (ci + n2 * dct1)) * rB * rB;
+  const double drs2 = ((-ct - n2 * dct2) * B - A * (ct + n2 * dct2)) * rB * rB;
+  const double drp1 = ((-ct - n1 * dct1) * D - C * (ct + n1 * dct1)) * rD * rD;
+  const double drp2 = ((ci - n1 * dct2) * D - C * (ci + n1 * dct2)) * rD * rD;
   const double dR1 = rs * drs1 + rp * drp1, dR2 = rs * drs2 + rp * drp2;
-  const double inv = reflected ? 2.0 / (rs * rs + rp * rp)
-                               : -1.0 / (2.0 * n1 * n2 * ci * ct * (1.0 / (B * B) + 1.0 / (D * D)));
+  const double inv = reflected ? 2.0 * rcp64(rs * rs + rp * rp)
+                               : -rcp64(2.0 * n1 * n2 * ci * ct * (rB * rB + rD * rD));
   d1 = static_cast<float>(dR1 * inv);
   d2 = static_cast<float>(dR2 * inv);
 }

@@ -439,8 +439,12 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
 // Scores are f32 per photon; the weighted sums are f64.
 // Work: per-set queues; a wave stays on one set until it runs out, so a lane
 // changes set rarely and flushes its f64 accumulators with few atomics.
-template <int kSurf>
-__global__ __launch_bounds__(1024) void adjoint_kernel(PhiloxKey key, uint64_t first_id,
+//
+// kSurf >= 0: every set has that surface model; kUni: a single set whose
+// parameters come as the kernel argument P0 (uniform, scalar registers).
+template <int kSurf, bool kUni>
+__global__ __launch_bounds__(1024) void adjoint_kernel(DevParams P0, PhiloxKey key,
+                                                       uint64_t first_id,
                                                        uint32_t steps_per_refill, uint32_t chunk,
                                                        AdjointArgs a) {
   const uint32_t lane = lane_id();
@@ -454,22 +458,40 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(PhiloxKey key, uint64_t f
   bool alive = false;
   uint32_t lset = 0xffffffffu, pset = 0;  // accumulator set, photon's set
   double g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0;
-  DevParams P{};
+  DevParams Pv{};
+  const DevParams& P = kUni ? P0 : Pv;
   Stream rng;
   Photon ph;
   AdjState adj{};
   uint32_t done = 0, fate = kMaxSteps;
 
-  auto flush = [&]() {
-    if (lset != 0xffffffffu) {
-      double* gr = a.grad + 5u * lset;
-      if (g0 != 0.0) atomicAdd(gr + 0, g0);
-      if (g1 != 0.0) atomicAdd(gr + 1, g1);
-      if (g2 != 0.0) atomicAdd(gr + 2, g2);
-      if (g3 != 0.0) atomicAdd(gr + 3, g3);
-      if (g4 != 0.0) atomicAdd(gr + 4, g4);
+  // Adds the accumulators of the lanes with `want` set to grad[lset], one
+  // wave-wide sum and one atomic per distinct set among them (uniform call).
+  auto flush = [&](bool want) {
+    want = want && lset != 0xffffffffu;
+    uint64_t pend = __ballot(want);
+    while (pend) {
+      const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(pend));
+      const uint32_t s = __builtin_amdgcn_readlane(lset, leader);
+      const bool mine = want && lset == s;
+      double v[5] = {mine ? g0 : 0.0, mine ? g1 : 0.0, mine ? g2 : 0.0, mine ? g3 : 0.0,
+                     mine ? g4 : 0.0};
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[j] += __shfl_xor(v[j], off, 64);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+          if (v[j] != 0.0) atomicAdd(a.grad + 5u * s + j, v[j]);
+      }
+      if (mine) {
+        g0 = g1 = g2 = g3 = g4 = 0.0;
+        want = false;
+      }
+      pend = __ballot(want);
     }
-    g0 = g1 = g2 = g3 = g4 = 0.0;
   };
 
   for (;;) {
@@ -509,10 +531,15 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(PhiloxKey key, uint64_t f
         }
       }
       const uint32_t r = rank_in(dead);
-      if (!alive && r < t1 + t2) {
-        pset = r < t1 ? s1 : ws;
+      const bool gets = !alive && r < t1 + t2;
+      const uint32_t nset = r < t1 ? s1 : ws;
+      // A lane moving to another set hands in its sums for the old one.
+      flush(gets && nset != lset);
+      if (gets && nset != lset) lset = nset;
+      if (gets) {
+        pset = nset;
         const uint64_t local = r < t1 ? b1 + r : b2 + (r - t1);
-        P = a.table[pset];
+        if (!kUni) Pv = a.table[pset];
         rng.start(first_id + a.offsets[pset] + local, key, 0u);
         emit(P, rng, ph);
         done = 0;
@@ -557,10 +584,6 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(PhiloxKey key, uint64_t f
           const uint32_t bx = time_bin(x, a.time_scale, a.time_bins);
           tterm = (wt - w[bx]) * static_cast<double>(ph.path / 299.792458f / P.tau);
         }
-        if (pset != lset) {
-          flush();
-          lset = pset;
-        }
         g0 += wt * static_cast<double>(s0);
         g1 += wt * static_cast<double>(s1f);
         g2 += wt * static_cast<double>(s2);
@@ -569,7 +592,7 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(PhiloxKey key, uint64_t f
       }
     }
   }
-  flush();
+  flush(true);
 }
 
 __global__ void tally_reduce_kernel(uint32_t words, int blocks, const uint32_t* partials,
@@ -725,13 +748,34 @@ int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds) {
   return blocks_per_cu(wavefront_kernel<false, true, false, kLambertian>, block, lds);
 }
 
-void launch_adjoint(PhiloxKey key, uint64_t first_id, const WavefrontShape& s,
-                    const AdjointArgs& a, hipStream_t stream) {
-  hipLaunchKernelGGL((adjoint_kernel<-1>), dim3(s.grid), dim3(s.block), 0, stream, key, first_id,
-                     s.steps_per_refill, s.queue_chunk, a);
+template <int S, bool U>
+void launch_adjoint_t(const DevParams& P0, PhiloxKey key, uint64_t first_id,
+                      const WavefrontShape& s, const AdjointArgs& a, hipStream_t stream) {
+  hipLaunchKernelGGL((adjoint_kernel<S, U>), dim3(s.grid), dim3(s.block), 0, stream, P0, key,
+                     first_id, s.steps_per_refill, s.queue_chunk, a);
 }
 
-int adjoint_blocks_per_cu(int block) { return blocks_per_cu(adjoint_kernel<-1>, block, 0); }
+template <bool U>
+void launch_adjoint_surf(int surface, const DevParams& P0, PhiloxKey key, uint64_t first_id,
+                         const WavefrontShape& s, const AdjointArgs& a, hipStream_t stream) {
+  switch (surface) {
+    case kPolished: launch_adjoint_t<kPolished, U>(P0, key, first_id, s, a, stream); break;
+    case kBlack: launch_adjoint_t<kBlack, U>(P0, key, first_id, s, a, stream); break;
+    case kSpecular: launch_adjoint_t<kSpecular, U>(P0, key, first_id, s, a, stream); break;
+    case kLambertian: launch_adjoint_t<kLambertian, U>(P0, key, first_id, s, a, stream); break;
+    default: launch_adjoint_t<-1, U>(P0, key, first_id, s, a, stream); break;
+  }
+}
+
+void launch_adjoint(const DevParams& P0, int surface, PhiloxKey key, uint64_t first_id,
+                    const WavefrontShape& s, const AdjointArgs& a, hipStream_t stream) {
+  if (a.n_events == 1)
+    launch_adjoint_surf<true>(surface, P0, key, first_id, s, a, stream);
+  else
+    launch_adjoint_surf<false>(surface, P0, key, first_id, s, a, stream);
+}
+
+int adjoint_blocks_per_cu(int block) { return blocks_per_cu(adjoint_kernel<-1, false>, block, 0); }
 
 size_t batch_lds_bytes(uint32_t n_events, uint32_t words) {
   const size_t b = static_cast<size_t>(n_events) * words * sizeof(uint32_t);
