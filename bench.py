@@ -127,9 +127,9 @@ def bench_differentiation():
     theta = cal.C4_NOMINAL.copy()
     cal.loss_and_grad(tpl, theta, T, n, n)  # warm
     loss, g, f_ms, a_ms, dl_dh = cal.loss_and_grad(tpl, theta, T, n, n)
-    nc = 400_000_000
-    lr, lr_se = cal.lr_gradient(tpl, theta, T, nc, n, dl_dh=dl_dh)
-    fd, fd_se = cal.fd_gradient(tpl, theta, T, nc, n, rel_step=1e-3, dl_dh=dl_dh)
+    nc = 4_000_000_000
+    lr, lr_se = cal.lr_gradient(tpl, theta, T, nc, n, parts=16, dl_dh=dl_dh)
+    fd, fd_se = cal.fd_gradient(tpl, theta, T, nc, n, rel_step=1e-3, parts=16, dl_dh=dl_dh)
     names = ("abs_length", "scat_length", "reflectivity", "n_det", "n_scint")
     check = {}
     for j, nm in enumerate(names):

@@ -234,6 +234,21 @@ __device__ __forceinline__ float rayleigh_cos(float u) {
   return __builtin_fminf(1.0f, __builtin_fmaxf(-1.0f, a - 1.0f / a));
 }
 
+// Tally binning (transport.hpp:170-181); `time_scale` = time_bins / time_max
+// and `two_hx`, `two_hy` = 2 hx, 2 hy, computed once on the host.
+__device__ __forceinline__ uint32_t time_bin(float t, float time_scale, uint32_t bins) {
+  const float f = t * time_scale;
+  return (f >= static_cast<float>(bins)) ? bins : static_cast<uint32_t>(f);
+}
+
+__device__ __forceinline__ uint32_t xy_bin(float x, float y, float hx, float hy, float two_hx,
+                                           float two_hy, uint32_t nb_bins) {
+  const float nb = static_cast<float>(nb_bins);
+  const uint32_t ix = static_cast<uint32_t>(__builtin_fminf((x + hx) / two_hx * nb, nb - 1.0f));
+  const uint32_t iy = static_cast<uint32_t>(__builtin_fminf((y + hy) / two_hy * nb, nb - 1.0f));
+  return iy * nb_bins + ix;
+}
+
 // ---------------------------------------------------------------------------
 // Photon state and the transport step.
 
@@ -246,9 +261,23 @@ struct Photon {
 // Likelihood-ratio scores a photon accumulates along its history (adjoint
 // kernel only): d log p(history) / d n_det and d n_scint from every Fresnel
 // decision, and the number of reflector hits (for d / d reflectivity).
+//
+// Readout decisions carry a baseline: with b = w[bin(t)], the weight the
+// photon would score if transmitted at this hit (known before the draw), the
+// decision contributes (g - b) s instead of g s, g the photon's final weight.
+// This is unbiased because E[s | state before the draw] = 0, and it makes the
+// transmitted branch contribute exactly (b - b) s = 0: near total internal
+// reflection that branch has s = -dR / (1 - R) ~ 1 / x with probability ~ x^(1/2)
+// (x the distance to the critical angle), an infinite-variance term whose
+// finite-sample means are visibly biased. Reflected readout decisions add s
+// to the score and b s to `base`; transmitted ones add nothing.
 struct AdjState {
   float s_ndet, s_nscint;
   uint32_t refl_hits;
+  double base_ndet, base_nscint;
+  const double* w;  // the set's weights, time_bins + 1 of them
+  float time_scale;
+  uint32_t time_bins;
 };
 
 // Score of one Fresnel decision: d log P(decision) / d n1 and / d n2 with
@@ -441,12 +470,18 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
   const bool decide = readout || polished || refl;
   const bool survived = decide && to_unit(rng.peek<2>()) <= prob;
   const bool lambert_reflect = refl && lambert && survived;
-  if (kAdj && scored) {
-    // Bernoulli score: d log R (reflected) or d log (1 - R) (transmitted).
+  if (kAdj && scored && (survived || !readout)) {
+    // Bernoulli score: d log R (reflected) or d log (1 - R) (transmitted);
+    // readout transmissions are skipped (their baseline term is zero).
     float d1, d2;
     fresnel_score(P.n_scint, readout ? P.n_det : P.n_out, cos_i, survived, d1, d2);
     adj->s_nscint += d1;
-    if (readout) adj->s_ndet += d2;
+    if (readout) {
+      adj->s_ndet += d2;
+      const double b = adj->w[time_bin(ph.t, adj->time_scale, adj->time_bins)];
+      adj->base_ndet += b * static_cast<double>(d2);
+      adj->base_nscint += b * static_cast<double>(d1);
+    }
   }
   if (kAdj && refl) ++adj->refl_hits;
 
@@ -489,21 +524,6 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
   info.survived = survived;
   info.lambert_reflect = lambert_reflect;
   return alive;
-}
-
-// Tally binning (transport.hpp:170-181); `time_scale` = time_bins / time_max
-// and `two_hx`, `two_hy` = 2 hx, 2 hy, computed once on the host.
-__device__ __forceinline__ uint32_t time_bin(float t, float time_scale, uint32_t bins) {
-  const float f = t * time_scale;
-  return (f >= static_cast<float>(bins)) ? bins : static_cast<uint32_t>(f);
-}
-
-__device__ __forceinline__ uint32_t xy_bin(float x, float y, float hx, float hy, float two_hx,
-                                           float two_hy, uint32_t nb_bins) {
-  const float nb = static_cast<float>(nb_bins);
-  const uint32_t ix = static_cast<uint32_t>(__builtin_fminf((x + hx) / two_hx * nb, nb - 1.0f));
-  const uint32_t iy = static_cast<uint32_t>(__builtin_fminf((y + hy) / two_hy * nb, nb - 1.0f));
-  return iy * nb_bins + ix;
 }
 
 }  // namespace ophip

@@ -544,7 +544,9 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(DevParams P0, PhiloxKey k
         emit(P, rng, ph);
         done = 0;
         fate = kMaxSteps;
-        adj = AdjState{0.0f, 0.0f, 0u};
+        adj = AdjState{0.0f, 0.0f, 0u, 0.0, 0.0,
+                       a.weights + static_cast<size_t>(pset) * (a.time_bins + 1u), a.time_scale,
+                       a.time_bins};
         alive = P.max_steps > 0u;
       }
     }
@@ -566,6 +568,12 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(DevParams P0, PhiloxKey k
           alive = false;
         }
       }
+      if (finished) {
+        // Readout baselines (see AdjState) apply to every finished photon;
+        // the other terms only to detected ones (undetected photons score 0).
+        g3 -= adj.base_ndet;
+        g4 -= adj.base_nscint;
+      }
       if (finished && fate == kDetected) {
         const float inf = __builtin_huge_valf();
         float s0 = 0.0f, s1f = 0.0f, s2 = 0.0f;
@@ -576,7 +584,7 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(DevParams P0, PhiloxKey k
         const uint32_t surf = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
         if (surf >= kSpecular) s2 = static_cast<float>(adj.refl_hits) / P.reflectivity;
         const uint32_t bt = time_bin(ph.t, a.time_scale, a.time_bins);
-        const double* w = a.weights + static_cast<size_t>(pset) * (a.time_bins + 1u);
+        const double* w = adj.w;
         const double wt = w[bt];
         double tterm = 0.0;
         if (P.tau > 0.0f) {

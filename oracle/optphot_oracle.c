@@ -363,11 +363,24 @@ static pstate start_photon(const oracle_params* p, stream_t* rng) {
   return s;
 }
 
+/* transport.hpp:170-173 */
+static inline uint32_t time_bin(float t, const oracle_tally_config* c) {
+  const float f = t * ((float)c->time_bins / c->time_max);
+  return (f >= (float)c->time_bins) ? c->time_bins : (uint32_t)f;
+}
+
 /* Likelihood-ratio scores of one history, in double (adjoint reference).
    The history itself is the FP32 one above; only the scores are f64. */
 typedef struct {
   double s_ndet, s_nscint;
   uint32_t refl_hits;
+  /* Readout baseline: b = w[bin(t)] at the hit (the weight if transmitted
+     now); reflected readout decisions add s to the score and b s here,
+     transmitted ones add nothing, which equals subtracting b s from g s for
+     every readout decision (unbiased: E[s | state] = 0). */
+  double base_ndet, base_nscint;
+  const double* w;
+  const oracle_tally_config* tc;
 } adjscore;
 
 /* d log P(decision) / d n1, d n2 for a Fresnel decision at cos_i (float
@@ -447,11 +460,14 @@ static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int
   if (face == 5) { /* readout +z, :108-118 */
     const float R = fresnel_R(p->n_scint, p->n_det, cos_i);
     const int refl = happens(stream_uniform(rng), R);
-    if (adj) {
+    if (adj && refl) {
       double d1, d2;
       fresnel_score(p->n_scint, p->n_det, cos_i, refl, &d1, &d2);
+      const double b = adj->w[time_bin(s->t, adj->tc)];
       adj->s_nscint += d1;
       adj->s_ndet += d2;
+      adj->base_nscint += b * d1;
+      adj->base_ndet += b * d2;
     }
     if (refl) {
       s->dir = reflect(s->dir, normal);
@@ -514,11 +530,7 @@ static void transport_photon(const oracle_params* p, uint64_t id, oracle_result*
   r->draws = stream_draws(&rng);
 }
 
-/* transport.hpp:170-181 */
-static inline uint32_t time_bin(float t, const oracle_tally_config* c) {
-  const float f = t * ((float)c->time_bins / c->time_max);
-  return (f >= (float)c->time_bins) ? c->time_bins : (uint32_t)f;
-}
+/* transport.hpp:176-181 */
 static inline uint32_t xy_bin(float x, float y, const oracle_params* p, const oracle_tally_config* c) {
   const float nb = (float)c->xy_bins;
   const uint32_t ix = (uint32_t)fminf((x + p->hx) / (2.0f * p->hx) * nb, nb - 1.0f);
@@ -716,10 +728,12 @@ ORACLE_EXPORT void oracle_adjoint(const oracle_params* params, const uint64_t* p
       stream_init(&rng, p->seed, first_id + offset + (uint64_t)i, 0u);
       pstate s = start_photon(p, &rng);
       int fate = ORACLE_FATE_MAX_STEPS;
-      adjscore adj = {0.0, 0.0, 0u};
+      adjscore adj = {0.0, 0.0, 0u, 0.0, 0.0, w, &tc};
       for (uint32_t step = 0; step < p->max_steps; ++step) {
         if (!step_photon_adj(p, &rng, &s, &fate, &adj)) break;
       }
+      g3 -= adj.base_ndet;
+      g4 -= adj.base_nscint;
       if (fate != ORACLE_FATE_DETECTED) continue;
       const double path = s.path;
       const double wt = w[time_bin(s.t, &tc)];
