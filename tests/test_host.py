@@ -1,0 +1,104 @@
+"""The C++ host layer (include/optphot_hip/{params,backend,config}.hpp): the
+reference's configuration keys, .npy + meta.json output and backend.hpp API,
+exercised through the compiled unit-test program and the optphot driver.
+
+CPU: parser / writer / value-type / argument-error checks (the reference's
+tests/test_config.cpp cases plus error paths). GPU: the backend.hpp functions
+through the C-ABI, and the driver's output files compared with the oracle
+photon by photon and word by word.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.configs import PARITY
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "gpu_optical_photons_amd", "bin")
+
+
+def _bin(name):
+    path = os.path.join(BIN, name)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built (run __graft_entry__.build())")
+    return path
+
+
+def test_host_unit_tests(tmp_path):
+    r = subprocess.run([_bin("test_host")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failed" in r.stdout
+
+
+def test_driver_help_and_cpu_backend_refused(tmp_path):
+    r = subprocess.run([_bin("optphot"), "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "size_x size_y size_z" in r.stdout
+    r = subprocess.run([_bin("optphot"), "--backend", "cpu"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "GPU backend only" in r.stderr
+    r = subprocess.run([_bin("optphot"), "--surface", "shiny"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "surface must be" in r.stderr
+
+
+def _cfg_args(p):
+    """optphot command-line settings reproducing SimParams p."""
+    surf = ("polished", "black", "specular", "lambertian")[p.surface]
+    src = ("point", "volume")[p.source]
+    a = ["--size_x", repr(2 * p.hx), "--size_y", repr(2 * p.hy), "--size_z", repr(2 * p.hz)]
+    for k in ("n_scint", "n_det", "n_out", "abs_length", "scat_length", "reflectivity", "src_x",
+              "src_y", "src_z", "tau"):
+        v = getattr(p, k)
+        a += [f"--{k}", "inf" if np.isinf(v) else repr(float(np.float32(v)))]
+    return a + ["--surface", surf, "--source", src, "--max_steps", str(p.max_steps),
+                "--seed", str(p.seed)]
+
+
+@pytest.mark.gpu
+def test_host_gpu_unit_tests(tmp_path):
+    r = subprocess.run([_bin("test_host"), "gpu"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["history", "wavefront"])
+@pytest.mark.parametrize("name", ["c1_plain", "c2_long", "every_process", "capped"])
+def test_driver_records_match_oracle(tmp_path, oracle_portable, name, kernel):
+    p = PARITY[name]
+    n = 20_000
+    out = tmp_path / "rec"
+    r = subprocess.run([_bin("optphot"), *_cfg_args(p), "--n_photons", str(n), "--mode",
+                        "records", "--kernel", kernel, "--out", str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = oracle_portable.records(p, n)
+    for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"):
+        a = np.load(out / f"{f}.npy")
+        assert a.dtype == ref[f].dtype and a.shape == (n,)
+        assert np.array_equal(a.view(np.uint8), ref[f].view(np.uint8)), f
+    meta = json.loads((out / "meta.json").read_text())
+    assert meta["n_photons"] == n and meta["seed"] == p.seed and meta["mode"] == "records"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["history", "wavefront"])
+def test_driver_tally_matches_oracle(tmp_path, oracle_portable, kernel):
+    p = PARITY["c2_long"]
+    n = 50_000
+    out = tmp_path / "tally"
+    r = subprocess.run([_bin("optphot"), *_cfg_args(p), "--n_photons", str(n), "--mode", "tally",
+                        "--time_bins", "256", "--time_max", "100", "--xy_bins", "64",
+                        "--kernel", kernel, "--out", str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = oracle_portable.tally(p, n, 256, 100.0, 64)
+    assert np.array_equal(np.load(out / "fate_counts.npy"), ref["fate_counts"])
+    assert np.array_equal(np.load(out / "time_hist.npy"), ref["time_hist"])
+    xy = np.load(out / "xy_hist.npy")
+    assert xy.shape == (64, 64)
+    assert np.array_equal(xy, np.asarray(ref["xy_hist"]).reshape(64, 64))
