@@ -192,7 +192,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(photon-id ranges would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would shard photon-id ranges; only one GPU is used here)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
