@@ -418,19 +418,63 @@ static void fresnel_score(float n1f, float n2f, float cif, int reflected, double
 
 /* transport.hpp:71-145. Returns 1 while alive, else sets *fate. `adj`
    (optional) collects the adjoint scores of the decisions taken. */
-static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int* fate,
-                           adjscore* adj);
-static int step_photon(const oracle_params* p, stream_t* rng, pstate* s, int* fate) {
-  return step_photon_adj(p, rng, s, fate, NULL);
+/* Per-step decision trace (test instrumentation): what the step did and how
+   close its deciding comparisons came to going the other way. */
+typedef struct {
+  uint8_t code;  /* 0 absorb, 1 scatter, 2 + face boundary; | 0x80 if a Bernoulli
+                    decision at the face came out true (reflected / survived) */
+  float margin;  /* smallest relative gap of the comparisons that decided */
+} steptrace;
+
+static inline float rel_gap(float a, float b) {
+  if (is_inf(a) || is_inf(b)) return INFINITY;
+  const float m = fmaxf(fabsf(a), fabsf(b));
+  return m > 0.0f ? fabsf(a - b) / m : 0.0f;
+}
+
+/* gap between the nearest and the second-nearest face along d (face choice) */
+static float face_choice_gap(v3 p, v3 d, float hx, float hy, float hz) {
+  float t[3] = {INFINITY, INFINITY, INFINITY};
+  const float pc[3] = {p.x, p.y, p.z}, dc[3] = {d.x, d.y, d.z}, h[3] = {hx, hy, hz};
+  for (int a = 0; a < 3; ++a)
+    if (dc[a] != 0.0f) t[a] = ((dc[a] > 0.0f ? h[a] : -h[a]) - pc[a]) / dc[a];
+  float lo = fminf(t[0], fminf(t[1], t[2]));
+  float second = INFINITY;
+  int used = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (!used && t[a] == lo) {
+      used = 1;
+      continue;
+    }
+    second = fminf(second, t[a]);
+  }
+  return rel_gap(lo, second);
 }
 
 static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int* fate,
-                           adjscore* adj) {
+                           adjscore* adj, steptrace* tr);
+static int step_photon(const oracle_params* p, stream_t* rng, pstate* s, int* fate) {
+  return step_photon_adj(p, rng, s, fate, NULL, NULL);
+}
+
+static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int* fate,
+                           adjscore* adj, steptrace* tr) {
   const float inv_speed = p->n_scint / kSpeedOfLight;
   const float s_abs = sample_exponential(p->abs_length, stream_uniform(rng));
   const float s_sca = sample_exponential(p->scat_length, stream_uniform(rng));
   int face;
   const float dist = box_exit(s->pos, s->dir, p->hx, p->hy, p->hz, &face);
+  if (tr) {
+    float m = fminf(rel_gap(s_abs, dist), rel_gap(s_sca, dist));
+    if (fminf(s_abs, s_sca) < dist) m = fminf(m, rel_gap(s_abs, s_sca));
+    if (!(s_abs < dist && s_abs <= s_sca) && !(s_sca < dist)) {
+      m = fminf(m, face_choice_gap(s->pos, s->dir, p->hx, p->hy, p->hz));
+      tr->code = (uint8_t)(2 + face);
+    } else {
+      tr->code = (s_abs < dist && s_abs <= s_sca) ? 0 : 1;
+    }
+    tr->margin = m;
+  }
 
   if (s_abs < dist && s_abs <= s_sca) { /* bulk absorption, :81-87 */
     s->pos = v3_add(s->pos, v3_scale(s_abs, s->dir));
@@ -459,7 +503,12 @@ static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int
 
   if (face == 5) { /* readout +z, :108-118 */
     const float R = fresnel_R(p->n_scint, p->n_det, cos_i);
-    const int refl = happens(stream_uniform(rng), R);
+    const float u = stream_uniform(rng);
+    const int refl = happens(u, R);
+    if (tr) {
+      tr->margin = fminf(tr->margin, fabsf(u - R));
+      if (refl) tr->code |= 0x80;
+    }
     if (adj && refl) {
       double d1, d2;
       fresnel_score(p->n_scint, p->n_det, cos_i, refl, &d1, &d2);
@@ -478,7 +527,12 @@ static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int
   }
   if (p->surface == ORACLE_SURFACE_POLISHED) { /* :120-128 */
     const float R = fresnel_R(p->n_scint, p->n_out, cos_i);
-    const int refl = happens(stream_uniform(rng), R);
+    const float u = stream_uniform(rng);
+    const int refl = happens(u, R);
+    if (tr) {
+      tr->margin = fminf(tr->margin, fabsf(u - R));
+      if (refl) tr->code |= 0x80;
+    }
     if (adj) {
       double d1, d2;
       fresnel_score(p->n_scint, p->n_out, cos_i, refl, &d1, &d2);
@@ -496,7 +550,12 @@ static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int
     return 0;
   }
   if (adj) ++adj->refl_hits;
-  if (!happens(stream_uniform(rng), p->reflectivity)) { /* :134-137 */
+  const float u_r = stream_uniform(rng);
+  if (tr) {
+    tr->margin = fminf(tr->margin, fabsf(u_r - p->reflectivity));
+    if (happens(u_r, p->reflectivity)) tr->code |= 0x80;
+  }
+  if (!happens(u_r, p->reflectivity)) { /* :134-137 */
     *fate = ORACLE_FATE_ABSORBED_SURFACE;
     return 0;
   }
@@ -607,6 +666,39 @@ ORACLE_EXPORT int oracle_max_threads(void) { return thread_count(0); }
 
 /* cpu_backend.cpp:88-113: records for photons first_id .. first_id+n-1.
    `draws` (optional) receives each photon's total Philox word count. */
+/* Step-by-step decision trace of photon `id` (test instrumentation): for each
+   of the first `cap` steps, the step's outcome code and the smallest relative
+   margin of the comparisons that decided it (see steptrace). Returns the
+   number of steps the photon took (may exceed cap). With `state` non-NULL it
+   also receives position, direction and time after each step (7 floats). Used to show that a
+   photon whose discrete history differs between two math builds diverged at
+   a near-tie, i.e. through last-bit rounding. */
+ORACLE_EXPORT uint32_t oracle_trace(const oracle_params* p, uint64_t id, uint32_t cap,
+                                    uint8_t* codes, float* margins, float* state) {
+  stream_t rng;
+  stream_init(&rng, p->seed, id, 0u);
+  pstate s = start_photon(p, &rng);
+  int fate = ORACLE_FATE_MAX_STEPS;
+  uint32_t step = 0;
+  while (step < p->max_steps) {
+    steptrace tr = {0, INFINITY};
+    const int alive = step_photon_adj(p, &rng, &s, &fate, NULL, &tr);
+    if (step < cap) {
+      codes[step] = tr.code;
+      margins[step] = tr.margin;
+      if (state) {
+        float* o = state + 7 * (size_t)step;
+        o[0] = s.pos.x, o[1] = s.pos.y, o[2] = s.pos.z;
+        o[3] = s.dir.x, o[4] = s.dir.y, o[5] = s.dir.z;
+        o[6] = s.t;
+      }
+    }
+    ++step;
+    if (!alive) break;
+  }
+  return step;
+}
+
 ORACLE_EXPORT void oracle_run_records(const oracle_params* p, uint64_t n, uint64_t first_id,
                                       int threads, uint8_t* fate, uint32_t* n_boundary,
                                       uint32_t* n_scatter, float* t, float* x, float* y, float* z,
@@ -730,7 +822,7 @@ ORACLE_EXPORT void oracle_adjoint(const oracle_params* params, const uint64_t* p
       int fate = ORACLE_FATE_MAX_STEPS;
       adjscore adj = {0.0, 0.0, 0u, 0.0, 0.0, w, &tc};
       for (uint32_t step = 0; step < p->max_steps; ++step) {
-        if (!step_photon_adj(p, &rng, &s, &fate, &adj)) break;
+        if (!step_photon_adj(p, &rng, &s, &fate, &adj, NULL)) break;
       }
       g3 -= adj.base_ndet;
       g4 -= adj.base_nscint;

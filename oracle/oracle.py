@@ -131,6 +131,23 @@ class Oracle(_Lib):
         self._fn("adjoint", None, [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.c_uint64,
                                    ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_int])
+        self._fn("trace", ctypes.c_uint32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                            _u8p, _f32p, ctypes.c_void_p])
+
+    def step_trace(self, params, photon_id, cap=1 << 14, with_state=False):
+        """(codes u8 [steps], margins f32 [steps]) of one photon: per step the
+        outcome (0 absorb, 1 scatter, 2 + face boundary, | 0x80 if the face's
+        Bernoulli decision came out true) and the smallest relative margin of
+        the comparisons that decided it."""
+        codes = np.zeros(cap, np.uint8)
+        margins = np.zeros(cap, np.float32)
+        state = np.zeros((cap, 7), np.float32) if with_state else None
+        n = int(self.trace(ctypes.byref(params), photon_id, cap, codes, margins,
+                           state.ctypes.data_as(ctypes.c_void_p) if with_state else None))
+        n = min(n, cap)
+        if with_state:
+            return codes[:n], margins[:n], state[:n]
+        return codes[:n], margins[:n]
 
     def gradient(self, params_list, photons, time_bins, time_max, weights, first_id=0,
                  threads=0):
