@@ -66,12 +66,29 @@ class _RecordsC(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in RECORD_FIELDS]
 
 
+def _share_torch_runtime():
+    """PyTorch-ROCm wheels ship their own HIP and HSA runtimes under the same
+    sonames (libamdhip64.so.7, libhsa-runtime64.so.1). Loading torch first
+    makes this library bind to those already-loaded runtimes, so the process
+    has one HIP runtime and torch tensors and this library's kernels share a
+    device and its memory. Loading this library first would pull in the
+    system runtime as well, and torch would then see no device. Skipped when
+    torch is absent or OPTPHOT_NO_TORCH=1."""
+    if os.environ.get("OPTPHOT_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is not built: the HIP library is required (there is no CPU fallback). "
             "Build it with `python -c 'import __graft_entry__ as g; g.build()'` or "
             "`make -C gpu_optical_photons_amd/csrc`.")
+    _share_torch_runtime()
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.POINTER
     sig = {
@@ -99,6 +116,10 @@ def _load():
                                                          ctypes.c_int, ctypes.c_uint64,
                                                          ctypes.c_uint64, P(EventStats),
                                                          P(RunTiming)]),
+        "optphot_run_gpu_records_device": (ctypes.c_int, [P(SimParams), ctypes.c_uint64,
+                                                          P(_RecordsC), ctypes.c_uint32,
+                                                          ctypes.c_int, ctypes.c_uint64,
+                                                          P(EventStats), P(RunTiming)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -251,6 +272,27 @@ def run_gpu_records_event(p: SimParams, n: int, steps_per_launch: int, block_siz
                                             ctypes.byref(st), ctypes.byref(tm))
     _check(rc)
     return rec, tm, st
+
+
+def run_gpu_records_device(p: SimParams, n: int, steps_per_launch: int = 0,
+                           block_size: int = 256, first_id: int = 0, device: str = "cuda:0"):
+    """Records written straight into device memory (new; no reference
+    counterpart): returns ({field: torch tensor on `device`}, RunTiming,
+    EventStats). steps_per_launch 0 = history kernel, >= 1 = wavefront kernel.
+    torch only allocates the device arrays; the library's kernels fill them."""
+    import torch
+    dtypes = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint32): torch.uint32,
+              np.dtype(np.float32): torch.float32}
+    out = {f: torch.empty(n, dtype=dtypes[np.dtype(d)], device=device)
+           for f, d in zip(RECORD_FIELDS, RECORD_DTYPES)}
+    torch.cuda.synchronize(device)  # allocations complete before another stream writes
+    c = _RecordsC(*[out[f].data_ptr() for f in RECORD_FIELDS])
+    tm, st = RunTiming(), EventStats()
+    rc = _lib.optphot_run_gpu_records_device(ctypes.byref(p), n, ctypes.byref(c),
+                                             steps_per_launch, block_size, first_id,
+                                             ctypes.byref(st), ctypes.byref(tm))
+    _check(rc)
+    return out, tm, st
 
 
 def run_gpu_tally_wavefront(p: SimParams, c: TallyConfig, n: int, first_id: int = 0,

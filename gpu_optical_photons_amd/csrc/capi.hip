@@ -11,10 +11,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/optphot_hip.h"
@@ -66,6 +70,86 @@ struct DevBuf {
   }
 };
 
+// Grow-only pinned host buffer (hipHostMalloc), the staging ring of the
+// records copy-out.
+struct PinnedBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t want) {
+    if (want <= bytes && ptr) return false;
+    if (ptr) HIPCHK(hipHostFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    HIPCHK(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+    bytes = want;
+    return true;
+  }
+};
+
+// A few host threads that copy staged records into the caller's arrays.
+class CopyPool {
+ public:
+  explicit CopyPool(unsigned n) {
+    for (unsigned i = 0; i < n; ++i) workers_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> l(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+  size_t size() const { return workers_.size(); }
+  // Runs every task (on the workers and the calling thread) and returns when
+  // all are done.
+  void run(std::vector<std::function<void()>>& tasks) {
+    if (tasks.empty()) return;
+    {
+      std::lock_guard<std::mutex> l(m_);
+      tasks_ = &tasks;
+      next_ = 0;
+      pending_ = tasks.size();
+    }
+    cv_.notify_all();
+    drain();
+    std::unique_lock<std::mutex> l(m_);
+    done_cv_.wait(l, [this] { return pending_ == 0; });
+    tasks_ = nullptr;
+  }
+
+ private:
+  void drain() {
+    for (;;) {
+      std::function<void()>* t = nullptr;
+      {
+        std::lock_guard<std::mutex> l(m_);
+        if (!tasks_ || next_ >= tasks_->size()) return;
+        t = &(*tasks_)[next_++];
+      }
+      (*t)();
+      std::lock_guard<std::mutex> l(m_);
+      if (--pending_ == 0) done_cv_.notify_all();
+    }
+  }
+  void loop() {
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [this] { return stop_ || (tasks_ && next_ < tasks_->size()); });
+        if (stop_) return;
+      }
+      drain();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex m_;
+  std::condition_variable cv_, done_cv_;
+  std::vector<std::function<void()>>* tasks_ = nullptr;
+  size_t next_ = 0, pending_ = 0;
+  bool stop_ = false;
+};
+
 struct Context {
   std::mutex mu;
   bool ready = false;
@@ -74,8 +158,13 @@ struct Context {
   size_t max_lds = 0;
   std::string name;
   hipStream_t stream = nullptr;
+  hipStream_t copy_stream = nullptr;  // records copy-out, overlapping the kernels
   hipEvent_t ev[4] = {};
   DevBuf records[8];  // fate, nb, ns, t, x, y, z, path
+  static constexpr int kStageSlots = 3;
+  PinnedBuf stage[kStageSlots];
+  std::vector<hipEvent_t> piece_ev;  // records runs: launch start / end, copy done
+  std::unique_ptr<CopyPool> pool;
   DevBuf partials, scratch, tally;
   DevBuf batch_table, batch_offsets, batch_counts;
   DevBuf adj_weights, adj_grad, adj_queue;
@@ -96,8 +185,21 @@ struct Context {
     name = std::string(prop.name) + " (" + prop.gcnArchName + ", " + std::to_string(cu_count) +
            " CUs)";
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    pool = std::make_unique<CopyPool>(std::min(8u, std::max(1u, hw / 2)) - 1);
     ready = true;
+  }
+
+  // Events of a records run with up to `pieces` pieces: a start / end pair
+  // per launch, then one copy-done event per piece.
+  void ensure_piece_events(size_t pieces) {
+    while (piece_ev.size() < 3 * pieces) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      piece_ev.push_back(e);
+    }
   }
 
   float elapsed(int a, int b) {
@@ -195,16 +297,6 @@ DevRecords ensure_records(Context& c, uint64_t cap, double& alloc_ms) {
                     c.records[2].as<uint32_t>(), c.records[3].as<float>(),
                     c.records[4].as<float>(),    c.records[5].as<float>(),
                     c.records[6].as<float>(),    c.records[7].as<float>()};
-}
-
-void copy_records(Context& c, const optphot_records* out, uint64_t offset, uint64_t m) {
-  void* dst[8] = {out->fate + offset, out->n_boundary + offset, out->n_scatter + offset,
-                  out->t + offset,    out->x + offset,          out->y + offset,
-                  out->z + offset,    out->path + offset};
-  const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
-  for (int i = 0; i < 8; ++i)
-    HIPCHK(hipMemcpyAsync(dst[i], c.records[i].ptr, m * sz[i], hipMemcpyDeviceToHost, c.stream));
-  HIPCHK(hipStreamSynchronize(c.stream));
 }
 
 void validate_records(const optphot_records* out) {
@@ -324,6 +416,183 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   return s;
 }
 
+// ---- records pipeline ------------------------------------------------------
+//
+// Photons first_id .. first_id+n-1 are transported in chunks of at most
+// `chunk` photons (the reference's bound on device memory, backend.hpp:86-89).
+// The wavefront kernel runs one launch per chunk (the reference's launch
+// count, gpu_event.cu:174-244); the history kernel splits chunks above 4M
+// photons into a few launches. Kernels write into the device record buffers, or
+// straight into the caller's device arrays. For host output each piece is
+// copied on c.copy_stream into pinned slot k % kStageSlots once the launch
+// that produced it has finished, and the host copies that slot into the
+// caller's arrays (several threads) while the GPU computes and copies later
+// pieces. A new chunk's kernels wait for the previous chunk's copies, since
+// they reuse the device buffers.
+
+enum class RecKernel { History, Wavefront };
+
+constexpr size_t kRecordBytes = 29;                                  // backend.hpp:49
+constexpr uint64_t kStagePhotonsMax = (32ull << 20) / kRecordBytes;  // 32 MB per slot
+constexpr uint64_t kPieceMin = 1ull << 16;
+constexpr uint64_t kWholeLaunchMax = 4ull << 20;
+
+struct RecordsRun {
+  uint64_t launches = 0;
+  double kernel_ms = 0.0;
+};
+
+DevRecords offset_records(const DevRecords& r, uint64_t k) {
+  return DevRecords{r.fate + k, r.nb + k, r.ns + k, r.t + k, r.x + k, r.y + k, r.z + k,
+                    r.path + k};
+}
+
+RecordsRun run_records(Context& c, const optphot_sim_params& p, uint64_t n,
+                       const optphot_records* host, const DevRecords* dev, RecKernel kind,
+                       int block, uint32_t steps, uint64_t first_id, uint64_t chunk,
+                       unsigned long long* scratch, optphot_run_timing& tm) {
+  RecordsRun run;
+  if (n == 0) return run;
+  const uint64_t limit = uint64_t{1} << 31;
+  const uint64_t cap = dev ? std::min(n, limit)
+                           : std::max<uint64_t>(1, std::min<uint64_t>({n, chunk, limit}));
+  // copy piece; a history launch covers one piece, a wavefront launch a chunk
+  uint64_t piece = cap;
+  if (host) {
+    piece = std::max<uint64_t>(kPieceMin, (cap + 7) / 8);
+    piece = std::min<uint64_t>({(piece + 63) / 64 * 64, kStagePhotonsMax, cap});
+  }
+  // Photons per launch: the wavefront kernel runs a chunk per launch; the
+  // history kernel splits a large chunk into a few launches (a multiple of the
+  // copy piece) so that copies start early, but each launch ends with its
+  // longest history, so small chunks stay whole.
+  uint64_t kpiece = cap;
+  if (kind == RecKernel::History && cap > kWholeLaunchMax) {
+    kpiece = std::max<uint64_t>(cap / 4, kWholeLaunchMax / 2);
+    kpiece = std::min<uint64_t>((kpiece + piece - 1) / piece * piece, cap);
+  }
+  DevRecords buf{};
+  if (!dev) buf = ensure_records(c, cap, tm.alloc_ms);
+  if (host) {
+    const auto ta = std::chrono::steady_clock::now();
+    bool grew = false;
+    for (auto& st : c.stage) grew |= st.ensure(piece * kRecordBytes);
+    if (grew) tm.alloc_ms += ms_since(ta);
+  }
+  const uint64_t chunks = (n + cap - 1) / cap;
+  const size_t max_pieces = static_cast<size_t>(chunks * ((cap + piece - 1) / piece));
+  // events: [0, 2 * max_pieces) launch start / end, then one per copy piece
+  c.ensure_piece_events(max_pieces);
+  hipEvent_t* kev = c.piece_ev.data();
+  hipEvent_t* cev = c.piece_ev.data() + 2 * max_pieces;
+
+  const DevParams P = to_dev(p);
+  const PhiloxKey key = key_of(p);
+  LaunchOut lo{};
+  lo.records = true;
+  lo.counters = scratch ? scratch + kScratchCounters : nullptr;
+  const optphot_wavefront_tuning tu{block, steps, 256, 0};
+
+  const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
+  struct Piece {
+    uint64_t off, m, dev_off;
+    size_t launch;
+  };
+  std::vector<Piece> pieces;
+  pieces.reserve(max_pieces);
+  size_t drained = 0;
+  double d2h_ms = 0.0;
+
+  auto launch = [&](uint64_t off, uint64_t m, uint64_t dev_off) {
+    const size_t k = run.launches++;
+    lo.rec = dev ? offset_records(*dev, off) : offset_records(buf, dev_off);
+    HIPCHK(hipEventRecord(kev[2 * k], c.stream));
+    if (kind == RecKernel::History) {
+      const int grid = static_cast<int>((m + block - 1) / block);
+      launch_history(P, key, first_id + off, m, block, grid, lo, c.stream);
+    } else {
+      const WavefrontShape sm = wavefront_shape(c, &tu, true, false, 0, m);
+      launch_zero_u64(scratch + kScratchQueue, 1, c.stream);
+      launch_wavefront(P, key, first_id + off, m, sm, scratch + kScratchQueue, lo, c.stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(kev[2 * k + 1], c.stream));
+    return k;
+  };
+
+  // Host side of piece k: wait for its copy, then scatter the slot into the
+  // caller's arrays.
+  auto drain = [&](size_t k) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const Piece& pc = pieces[k];
+    HIPCHK(hipEventSynchronize(cev[k]));
+    const char* src = static_cast<const char*>(c.stage[k % Context::kStageSlots].ptr);
+    char* dst[8] = {reinterpret_cast<char*>(host->fate + pc.off),
+                    reinterpret_cast<char*>(host->n_boundary + pc.off),
+                    reinterpret_cast<char*>(host->n_scatter + pc.off),
+                    reinterpret_cast<char*>(host->t + pc.off),
+                    reinterpret_cast<char*>(host->x + pc.off),
+                    reinterpret_cast<char*>(host->y + pc.off),
+                    reinterpret_cast<char*>(host->z + pc.off),
+                    reinterpret_cast<char*>(host->path + pc.off)};
+    std::vector<std::function<void()>> tasks;
+    const size_t split = pc.m >= (uint64_t{1} << 18) ? c.pool->size() + 1 : 1;
+    for (int a = 0; a < 8; ++a) {
+      const size_t bytes = pc.m * sz[a];
+      const size_t part = (bytes + split - 1) / split;
+      for (size_t b = 0; b < bytes; b += part) {
+        const size_t len = std::min(part, bytes - b);
+        char* d = dst[a] + b;
+        const char* from = src + b;
+        tasks.emplace_back([d, from, len] { std::memcpy(d, from, len); });
+      }
+      src += piece * sz[a];
+    }
+    c.pool->run(tasks);
+    d2h_ms += ms_since(t0);
+  };
+
+  for (uint64_t ch = 0; ch < chunks; ++ch) {
+    const uint64_t c0 = ch * cap;
+    const uint64_t cm = std::min(cap, n - c0);
+    const size_t first_piece = pieces.size();
+    if (host && first_piece > 0)  // device buffers reused: previous chunk copied first
+      HIPCHK(hipStreamWaitEvent(c.stream, cev[first_piece - 1], 0));
+    size_t k = 0;
+    for (uint64_t o = 0; o < cm; o += piece) {
+      if (o % kpiece == 0) k = launch(c0 + o, std::min(kpiece, cm - o), o);
+      pieces.push_back(Piece{c0 + o, std::min(piece, cm - o), o, k});
+    }
+    if (!host) continue;
+    for (size_t k = first_piece; k < pieces.size(); ++k) {
+      const Piece& pc = pieces[k];
+      // slot k % kStageSlots is free once the host has copied out its last piece
+      while (drained + Context::kStageSlots <= k) drain(drained++);
+      HIPCHK(hipStreamWaitEvent(c.copy_stream, kev[2 * pc.launch + 1], 0));
+      char* slot = static_cast<char*>(c.stage[k % Context::kStageSlots].ptr);
+      const void* srcs[8] = {buf.fate + pc.dev_off, buf.nb + pc.dev_off, buf.ns + pc.dev_off,
+                             buf.t + pc.dev_off,    buf.x + pc.dev_off,  buf.y + pc.dev_off,
+                             buf.z + pc.dev_off,    buf.path + pc.dev_off};
+      for (int a = 0; a < 8; ++a) {
+        HIPCHK(hipMemcpyAsync(slot, srcs[a], pc.m * sz[a], hipMemcpyDeviceToHost,
+                              c.copy_stream));
+        slot += piece * sz[a];
+      }
+      HIPCHK(hipEventRecord(cev[k], c.copy_stream));
+    }
+  }
+  while (host && drained < pieces.size()) drain(drained++);
+  HIPCHK(hipStreamSynchronize(c.stream));
+  for (size_t k = 0; k < run.launches; ++k) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, kev[2 * k], kev[2 * k + 1]));
+    run.kernel_ms += ms;
+  }
+  tm.kernel_ms += run.kernel_ms;
+  tm.d2h_ms += d2h_ms;
+  return run;
+}
+
 void validate_batch(const optphot_sim_params* params, const uint64_t* photons, uint32_t n_events,
                     uint32_t time_bins, float time_max, uint32_t* out, int method) {
   if (!params || !photons || !out) throw InvalidArgument("batch: NULL argument");
@@ -416,26 +685,56 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records(
     c.init();
     optphot_run_timing tm{};
     const auto t0 = std::chrono::steady_clock::now();
-    const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(n, chunk ? chunk : 1));
-    const DevRecords rec = ensure_records(c, cap, tm.alloc_ms);
-    unsigned long long* scratch = ensure_scratch(c, 1, tm.alloc_ms);
-    LaunchOut lo{};
-    lo.rec = rec;
-    lo.records = true;
-    lo.counters = nullptr;
-    (void)scratch;
-    const DevParams P = to_dev(*p);
-    for (uint64_t done = 0; done < n; done += cap) {
-      const uint64_t m = std::min(cap, n - done);
-      const int grid = static_cast<int>((m + block_size - 1) / block_size);
-      HIPCHK(hipEventRecord(c.ev[0], c.stream));
-      launch_history(P, key_of(*p), first_id + done, m, block_size, grid, lo, c.stream);
+    run_records(c, *p, n, out, nullptr, RecKernel::History, block_size, 0, first_id, chunk,
+                nullptr, tm);
+    tm.total_ms = ms_since(t0);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_records_device(
+    const optphot_sim_params* p, uint64_t n, const optphot_records* dev_out,
+    uint32_t steps_per_launch, int block_size, uint64_t first_id, optphot_event_stats* stats,
+    optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    validate_block(block_size, "device records");
+    if (n > 0) validate_records(dev_out);
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    if (n > 0) {
+      hipPointerAttribute_t attr{};
+      if (hipPointerGetAttributes(&attr, dev_out->t) != hipSuccess ||
+          attr.type != hipMemoryTypeDevice)
+        throw InvalidArgument("device records: output arrays must be device memory");
+    }
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+    const DevRecords d{dev_out->fate, dev_out->n_boundary, dev_out->n_scatter, dev_out->t,
+                       dev_out->x,    dev_out->y,          dev_out->z,         dev_out->path};
+    unsigned long long* scratch = nullptr;
+    if (steps_per_launch > 0) {
+      optphot_wavefront_tuning tu{block_size, steps_per_launch, 256, 0};
+      const WavefrontShape s = wavefront_shape(c, &tu, true, false, 0, std::max<uint64_t>(n, 1));
+      scratch = ensure_scratch(c, s.grid, tm.alloc_ms);
+      launch_zero_u64(scratch, kScratchSums, c.stream);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c.ev[1], c.stream));
-      tm.kernel_ms += c.elapsed(0, 1);
-      const auto td = std::chrono::steady_clock::now();
-      copy_records(c, out, done, m);
-      tm.d2h_ms += ms_since(td);
+    }
+    const RecordsRun r = run_records(
+        c, *p, n, nullptr, &d, steps_per_launch > 0 ? RecKernel::Wavefront : RecKernel::History,
+        block_size, steps_per_launch, first_id, 0, scratch, tm);
+    if (stats) {
+      optphot_event_stats st{};
+      st.launches = r.launches;
+      st.kernels_ms = r.kernel_ms;
+      if (scratch) {
+        optphot_step_counters cnt{};
+        read_counters(c, &cnt);
+        st.useful_lane_steps = cnt.v[kCntUseful];
+        st.issued_lane_steps = cnt.v[kCntIssued];
+      }
+      *stats = st;
     }
     tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
@@ -505,38 +804,17 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records_event(
     std::lock_guard<std::mutex> lock(c.mu);
     c.init();
     optphot_run_timing tm{};
-    optphot_event_stats st{};
     const auto t0 = std::chrono::steady_clock::now();
-    const uint64_t cap = std::max<uint64_t>(
-        1, std::min<uint64_t>({n, chunk ? chunk : 1, uint64_t{1} << 31}));
-    const DevRecords rec = ensure_records(c, cap, tm.alloc_ms);
     optphot_wavefront_tuning tu{block_size, steps_per_launch, 256, 0};
-    const WavefrontShape s = wavefront_shape(c, &tu, true, false, 0, cap);
+    const WavefrontShape s = wavefront_shape(c, &tu, true, false, 0, std::max<uint64_t>(n, 1));
     unsigned long long* scratch = ensure_scratch(c, s.grid, tm.alloc_ms);
-    LaunchOut lo{};
-    lo.rec = rec;
-    lo.records = true;
-    lo.counters = scratch + kScratchCounters;
-    const DevParams P = to_dev(*p);
     launch_zero_u64(scratch, kScratchSums, c.stream);  // queue + counters
     HIPCHK(hipGetLastError());
-    for (uint64_t done = 0; done < n; done += cap) {
-      const uint64_t m = std::min(cap, n - done);
-      const WavefrontShape sm = wavefront_shape(c, &tu, true, false, 0, m);
-      launch_zero_u64(scratch + kScratchQueue, 1, c.stream);
-      HIPCHK(hipEventRecord(c.ev[0], c.stream));
-      launch_wavefront(P, key_of(*p), first_id + done, m, sm, scratch + kScratchQueue, lo,
-                       c.stream);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c.ev[1], c.stream));
-      const float k = c.elapsed(0, 1);
-      tm.kernel_ms += k;
-      st.kernels_ms += k;
-      ++st.launches;
-      const auto td = std::chrono::steady_clock::now();
-      copy_records(c, out, done, m);
-      tm.d2h_ms += ms_since(td);
-    }
+    const RecordsRun r = run_records(c, *p, n, out, nullptr, RecKernel::Wavefront, block_size,
+                                     steps_per_launch, first_id, chunk, scratch, tm);
+    optphot_event_stats st{};
+    st.launches = r.launches;
+    st.kernels_ms = r.kernel_ms;
     optphot_step_counters cnt{};
     read_counters(c, &cnt);
     st.useful_lane_steps = cnt.v[kCntUseful];

@@ -126,7 +126,10 @@ const char* optphot_last_error(void);
 /* run_gpu_records(p, n, out, block_size=256, first_id=0, chunk=1<<24),
  * backend.hpp:88-89 / gpu_backend.cu:119-154: history-based kernel, one lane
  * per photon, records of photons first_id .. first_id+n-1 into `out`.
- * block_size: multiple of 64 in [64, 1024]. `timing` may be NULL. */
+ * block_size: multiple of 64 in [64, 1024]. `timing` may be NULL. The device
+ * holds `chunk` photons of records; each chunk runs as several launches whose
+ * records are copied out through pinned staging buffers while later launches
+ * run (d2h_ms: host time spent waiting for and copying records). */
 int optphot_run_gpu_records(const optphot_sim_params* p, uint64_t n, const optphot_records* out,
                             int block_size, uint64_t first_id, uint64_t chunk,
                             optphot_run_timing* timing);
@@ -149,6 +152,17 @@ int optphot_run_gpu_records_event(const optphot_sim_params* p, uint64_t n,
                                   optphot_event_stats* stats, optphot_run_timing* timing);
 
 /* ---- New entry points (no counterpart in the reference) -------------------- */
+
+/* Records of photons first_id .. first_id+n-1 written straight into
+ * caller-owned DEVICE arrays (8 arrays of n elements, as optphot_records),
+ * for callers that consume records on the GPU (e.g. torch tensors): no PCIe
+ * copy. steps_per_launch = 0 selects the history kernel, >= 1 the wavefront
+ * kernel with that many steps between refills. Same records, bit for bit, as
+ * optphot_run_gpu_records. stats / timing may be NULL. */
+int optphot_run_gpu_records_device(const optphot_sim_params* p, uint64_t n,
+                                   const optphot_records* dev_out, uint32_t steps_per_launch,
+                                   int block_size, uint64_t first_id, optphot_event_stats* stats,
+                                   optphot_run_timing* timing);
 
 /* Tally of photons first_id .. first_id+n-1 by the persistent wavefront kernel
  * (device work queue, refill of finished lanes, LDS-privatised histograms).

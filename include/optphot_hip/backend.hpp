@@ -203,6 +203,20 @@ inline RunTiming run_gpu_tally_wavefront(const SimParams& p, const TallyConfig& 
   return tm;
 }
 
+// Records written straight into caller-owned device arrays (n elements each,
+// e.g. hipMalloc'ed or a framework's tensors): no copy to the host.
+// steps_per_launch 0 = history kernel, >= 1 = wavefront kernel.
+inline RunTiming run_gpu_records_device(const SimParams& p, uint64_t n,
+                                        const optphot_records& device_out,
+                                        uint32_t steps_per_launch = 0, int block_size = 256,
+                                        uint64_t first_id = 0, EventStats* stats = nullptr) {
+  RunTiming tm;
+  detail::check(optphot_run_gpu_records_device(
+      c_params(p), n, &device_out, steps_per_launch, block_size, first_id,
+      reinterpret_cast<optphot_event_stats*>(stats), reinterpret_cast<optphot_run_timing*>(&tm)));
+  return tm;
+}
+
 // Per-event results of a batch: 5 fate counts, then the arrival-time
 // histogram of detected photons (time_bins + 1 bins, last = overflow).
 struct EventTally {

@@ -49,6 +49,35 @@ def test_records_first_id_and_chunk(backend, oracle_portable):
     assert np.array_equal(tail.t, a.t[1000:1500])
 
 
+def test_records_pipeline_many_pieces(backend, oracle_portable):
+    """Records copied out in pieces through the pinned staging ring, across
+    chunk boundaries (device buffers reused), for both kernels."""
+    p = configs.c1_plain()
+    n = 3_000_000
+    ref = oracle_portable.records(p, n)
+    print("[parity] records pipeline history", flush=True)
+    a, tm = backend.run_gpu_records(p, n, chunk=1_000_000)
+    assert_records_identical(a, ref, "history pipeline")
+    print("[parity] records pipeline wavefront", flush=True)
+    ev, _, st = backend.run_gpu_records_event(p, n, 4, chunk=1_000_000)
+    assert_records_identical(ev, ref, "wavefront pipeline")
+    assert st.launches == 3
+
+
+@pytest.mark.parametrize("k", [0, 1, 5])
+def test_device_records(backend, oracle_portable, k):
+    p = configs.c2_long()
+    n = 300_001
+    print(f"[parity] device records k={k}", flush=True)
+    out, tm, st = backend.run_gpu_records_device(p, n, steps_per_launch=k, first_id=17)
+    ref = oracle_portable.records(p, n, first_id=17)
+    host = {f: out[f].cpu().numpy() for f in out}
+    assert_records_identical(host, ref, f"device k={k}")
+    assert tm.d2h_ms == 0.0
+    if k:
+        assert st.useful_lane_steps > 0 and st.launches == 1
+
+
 @pytest.mark.parametrize("name", ["default", "wrapped_lambertian", "capped", "c2_long"])
 def test_event_records_equal_history_for_any_k(backend, name):
     p = configs.PARITY[name]
