@@ -188,6 +188,10 @@ def main():
     ap.add_argument("--photons", type=int, default=N_PHOTONS)
     ap.add_argument("--only-main", action="store_true",
                     help="only the long-histories line (no other workloads)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only run the library's VALU calibration kernel (all lanes, then odd "
+                         "lanes only): a known instruction count for reading rocprofv3 VALU "
+                         "counters")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (torch), the known-size "
                          "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -202,6 +206,29 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 4,
                           "copy_probe_bytes_written": b.numel() * 4, "copies": 3}))
+        return
+    if args.valu_probe:
+        import ctypes
+
+        from gpu_optical_photons_amd import backend
+        fn = backend.lib().optphot_valu_probe
+        fn.restype = ctypes.c_int
+        fn.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                       ctypes.POINTER(ctypes.c_double)]
+        blocks, iters = 2048, 1 << 15
+        out = {"blocks": blocks, "threads_per_block": 256, "iters": iters,
+               "fma_per_wave": 8 * iters}
+        for half in (0, 1):
+            ms = ctypes.c_double()
+            backend._check(fn(blocks, iters, half, ctypes.byref(ms)))  # warm-up
+            backend._check(fn(blocks, iters, half, ctypes.byref(ms)))
+            waves = blocks * 4
+            key = "half_lanes" if half else "all_lanes"
+            out[key] = {"kernel_ms": ms.value,
+                        "wave_instructions_per_s": waves * 8 * iters / (ms.value * 1e-3),
+                        "lane_fma_per_s": waves * 8 * iters * (32 if half else 64)
+                        / (ms.value * 1e-3)}
+        print(json.dumps(out))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -139,6 +139,11 @@ void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials
 // Fills n u64 words with zero (a kernel, so it may sit inside captured graphs).
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
 
+// VALU calibration kernel: `blocks` x 256 lanes, each iters x 8 FMAs (only odd
+// lanes when `half`); out holds blocks x 256 floats.
+void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
+                       hipStream_t stream);
+
 // Occupancy queries (blocks per CU for a block size and dynamic LDS bytes).
 int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds_bytes);

@@ -35,9 +35,15 @@ from __future__ import annotations
 
 import math
 
-# MI355X peaks (published): 256 CUs at 2.4 GHz.
-FP32_VECTOR_PEAK_TFLOPS = 157.3            # vendor FP32 vector figure (packed FMA)
-VALU_ISSUE_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # one lane-op per lane per clock = 39.3
+# MI355X peaks (published): 256 CUs at 2.4 GHz, FP32 vector 157.3 TFLOP/s
+# counting an FMA as 2 FLOPs. A CU executes 128 FP32 lane-operations per clock
+# without packed math: the library's VALU calibration kernel (8 independent
+# plain v_fma_f32 chains per lane) issues 9.0e11 wave64 instructions/s, i.e.
+# 1.46 per CU per clock at 2.4 GHz, more than a 64-lane-per-clock CU could.
+# So the ceiling for this code, which has one algorithmic op per instruction
+# (no contraction), is 256 x 128 x 2.4e9 = 78.6e12 lane-ops/s.
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+VALU_ISSUE_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12  # 78.6 lane-ops per second (x 1e12)
 HBM_PEAK_GBS = 8000.0
 
 DRAW = 98.0 / 4.0 + 4.0

@@ -219,6 +219,12 @@ int optphot_run_gpu_adjoint(const optphot_sim_params* params, const uint64_t* ph
                             float time_max, const double* weights, double* grad,
                             const optphot_wavefront_tuning* tuning, optphot_run_timing* timing);
 
+/* Calibration of profiler VALU counters: launches `blocks` x 256 lanes, each
+ * running iters x 8 independent v_fma_f32 (only the odd lanes of every wave
+ * when half_lanes != 0), and returns the kernel time. A wave executes
+ * iters x 8 VALU FMAs plus a handful of setup instructions. */
+int optphot_valu_probe(uint32_t blocks, uint32_t iters, int half_lanes, double* kernel_ms);
+
 /* Number of compute units and the kernels' register / occupancy facts,
  * written as "key=value" lines into buf (diagnostics). */
 int optphot_device_info(char* buf, size_t len);
