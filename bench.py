@@ -160,12 +160,14 @@ def bench_differentiation():
 def bench_calibration():
     """Batched differentiable calibration: 256 parameter sets x 1e6 photons,
     long-histories geometry, (abs_length, R, n_det) fitted by 50 Adam
-    iterations to targets from hidden parameters under a held-out seed."""
+    iterations to targets from hidden parameters under a held-out seed; each
+    iteration one captured graph (forward, chi-square weights, adjoint)."""
     from gpu_optical_photons_amd import calibrate as cal
     r = cal.calibrate(n_sets=256, photons=1_000_000, iters=50)
     return {k: r[k] for k in ("n_sets", "photons_per_set", "iterations", "median_rel_error",
-                              "p90_rel_error", "forward_ms", "adjoint_ms", "wall_s",
-                              "forward_photons_per_s", "fwd_adj_photons_per_s")} | {
+                              "p90_rel_error", "graph_ms", "forward_only_ms", "wall_s",
+                              "forward_photons_per_s", "fwd_adj_photons_per_s",
+                              "fwd_adj_over_fwd")} | {
         "final_median_loss": r["losses"][-1], "first_median_loss": r["losses"][0]}
 
 

@@ -74,6 +74,46 @@ def run_gpu_adjoint(batch: EventBatch, weights, time_bins: int, time_max: float,
     return grad, tm
 
 
+_step_fn = None
+
+
+def run_gpu_chi2_step(batch: EventBatch, target, target_photons, time_bins: int, time_max: float,
+                      first_id: int = 0, forward_tuning=None, adjoint_tuning=None):
+    """One calibration step in a single captured graph (optphot_run_gpu_chi2_step):
+    forward histograms, the chi-square loss of chi2_loss against `target`
+    ([E, time_bins + 1] fractions) and its adjoint gradient. Returns
+    (hist [E, T+1], loss [E], grad [E, 5], RunTiming)."""
+    global _step_fn
+    if _step_fn is None:
+        f = backend.lib().optphot_run_gpu_chi2_step
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64,
+                      ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                      ctypes.POINTER(backend.WavefrontTuning),
+                      ctypes.POINTER(backend.WavefrontTuning), ctypes.POINTER(backend.RunTiming)]
+        _step_fn = f
+    E = batch.n_events
+    arr = (SimParams * E)(*batch.params)
+    photons = np.ascontiguousarray(batch.photons, dtype=np.uint64)
+    T = np.ascontiguousarray(target, dtype=np.float64)
+    if T.shape != (E, time_bins + 1):
+        raise ValueError(f"target must have shape ({E}, {time_bins + 1})")
+    nt = np.ascontiguousarray(np.broadcast_to(np.asarray(target_photons, np.float64), (E,)))
+    hist = np.zeros((E, time_bins + 1), np.float64)
+    loss = np.zeros(E, np.float64)
+    grad = np.zeros((E, 5), np.float64)
+    tm = backend.RunTiming()
+    tunings = [backend.WavefrontTuning(**t) if isinstance(t, dict) else t
+               for t in (forward_tuning, adjoint_tuning)]
+    rc = _step_fn(ctypes.addressof(arr), photons.ctypes.data, E, first_id, time_bins, time_max,
+                  T.ctypes.data, nt.ctypes.data, hist.ctypes.data, loss.ctypes.data,
+                  grad.ctypes.data, *[ctypes.byref(t) if t is not None else None for t in tunings],
+                  ctypes.byref(tm))
+    backend._check(rc)
+    return hist, loss, grad, tm
+
+
 def make_function():
     """The torch.autograd.Function (torch is imported lazily)."""
     import torch

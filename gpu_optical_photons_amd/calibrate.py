@@ -13,8 +13,9 @@ Batched calibration: parameter sets with the long-histories geometry
 (100 x 100 x 50 mm, Lambertian walls, Rayleigh, volume source), hidden
 (abs_length, R, n_det) drawn from U[300, 3000] x U[0.85, 0.99] x U[1.4, 1.6],
 targets generated under a held-out seed; each set fits its three parameters
-by Adam on normalised coordinates, one forward (batched tally) + adjoint
-pass per iteration.
+by Adam on normalised coordinates. Each iteration is one captured hipGraph
+(optphot_run_gpu_chi2_step): the batched forward launch, the chi-square loss
+weights on the device, and the adjoint launch.
 """
 from __future__ import annotations
 
@@ -125,9 +126,12 @@ def c5_template(seed: int) -> SimParams:
 
 
 def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=0.08,
-              draw_seed=5, seed=1, target_seed=99, target_photons=None, log=None):
+              draw_seed=5, seed=1, target_seed=99, target_photons=None, log=None,
+              time_forward=True):
     """Fits (abs_length, R, n_det) of every set to its own target histogram.
-    Returns a dict with hidden / fitted parameters, errors and timings."""
+    Returns a dict with hidden / fitted parameters, errors and timings; with
+    time_forward, each iteration also times a forward-only batch at the same
+    parameters (outside the fit) for the forward+adjoint / forward ratio."""
     rng = np.random.default_rng(draw_seed)
     hidden = LO + (HI - LO) * rng.random((n_sets, 3))
     target_photons = target_photons or photons
@@ -149,22 +153,18 @@ def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=
     v = np.zeros_like(x)
     b1, b2, eps = 0.9, 0.999, 1e-8
     tpl = c5_template(seed)
-    fwd_ms = adj_ms = 0.0
+    step_ms = fwd_ms = 0.0
     wall0 = time.perf_counter()
     losses = []
     for it in range(1, iters + 1):
         th = theta_of(x)
         batch = EventBatch([adjoint.with_theta(tpl, r) for r in th],
                            np.full(n_sets, photons, np.uint64))
-        h, tf = adjoint.histograms(batch, bins, tmax)
-        f, F = h.sum(1, keepdims=True), T.sum(1, keepdims=True)
-        wk = 1.0 / (T + 1.0 / target_photons)
-        wf = 1.0 / (F * (1.0 - F) + 1.0 / target_photons)
-        loss = ((h - T) ** 2 * wk).sum(1) + ((f - F) ** 2 * wf)[:, 0]
-        dl_dh = 2.0 * (h - T) * wk + 2.0 * (f - F) * wf
-        g, ta = adjoint.run_gpu_adjoint(batch, dl_dh, bins, tmax)
-        fwd_ms += tf.kernel_ms
-        adj_ms += ta.kernel_ms
+        # one captured graph per iteration: forward, chi-square weights, adjoint
+        _, loss, g, ts = adjoint.run_gpu_chi2_step(batch, T, target_photons, bins, tmax)
+        step_ms += ts.kernel_ms
+        if time_forward:  # forward alone at the same parameters, for the cost ratio
+            fwd_ms += adjoint.histograms(batch, bins, tmax)[1].kernel_ms
         gx = g[:, list(FIT)] * (HI - LO)  # chain rule to normalised coordinates
         m = b1 * m + (1 - b1) * gx
         v = b2 * v + (1 - b2) * gx * gx
@@ -184,7 +184,8 @@ def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=
         "p90_rel_error": dict(zip(("abs_length", "reflectivity", "n_det"),
                                   np.percentile(rel_err, 90, axis=0).tolist())),
         "losses": losses,
-        "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "wall_s": wall,
-        "forward_photons_per_s": total / (fwd_ms * 1e-3),
-        "fwd_adj_photons_per_s": total / ((fwd_ms + adj_ms) * 1e-3),
+        "graph_ms": step_ms, "wall_s": wall, "forward_only_ms": fwd_ms,
+        "forward_photons_per_s": total / (fwd_ms * 1e-3) if fwd_ms else None,
+        "fwd_adj_photons_per_s": total / (step_ms * 1e-3),
+        "fwd_adj_over_fwd": step_ms / fwd_ms if fwd_ms else None,
     }

@@ -171,6 +171,19 @@ struct Context {
   // Captured graph of the batched launch and the arguments it was built for.
   hipGraphExec_t batch_exec = nullptr;
   std::string batch_key;
+  // Calibration step (forward + chi-square weights + adjoint) graph and buffers.
+  DevBuf step_target, step_inv_nt, step_hist, step_loss;
+  hipGraphExec_t step_exec = nullptr;
+  std::string step_key;
+
+  // Every captured graph refers to the buffers it was built with.
+  void drop_graphs() {
+    if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
+    if (step_exec) (void)hipGraphExecDestroy(step_exec);
+    batch_exec = step_exec = nullptr;
+    batch_key.clear();
+    step_key.clear();
+  }
 
   void init() {
     if (ready) return;
@@ -310,7 +323,10 @@ constexpr size_t kScratchQueue = 0, kScratchCounters = 1, kScratchSums = 1 + kCn
 
 unsigned long long* ensure_scratch(Context& c, int grid, double& alloc_ms) {
   const auto t0 = std::chrono::steady_clock::now();
-  if (c.scratch.ensure((kScratchSums + 2 * static_cast<size_t>(grid)) * 8)) alloc_ms += ms_since(t0);
+  if (c.scratch.ensure((kScratchSums + 2 * static_cast<size_t>(grid)) * 8)) {
+    alloc_ms += ms_since(t0);
+    c.drop_graphs();  // captured graphs hold the old scratch address
+  }
   return c.scratch.as<unsigned long long>();
 }
 
@@ -885,11 +901,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     grew |= c.batch_counts.ensure(zero_u64 * 8);
     if (grew) {
       tm.alloc_ms += ms_since(ta);
-      if (c.batch_exec) {  // buffers moved: the captured graph is stale
-        HIPCHK(hipGraphExecDestroy(c.batch_exec));
-        c.batch_exec = nullptr;
-        c.batch_key.clear();
-      }
+      c.drop_graphs();  // buffers moved: captured graphs are stale
     }
     const WavefrontShape s = [&] {
       const int max_per_cu =
@@ -939,7 +951,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
                                "/" + std::to_string(words) + "/" + std::to_string(lo.evt.time_scale) +
                                "/" + std::to_string(s.block) + "/" + std::to_string(s.grid) + "/" +
                                std::to_string(s.steps_per_refill) + "/" +
-                               std::to_string(s.queue_chunk);
+                               std::to_string(s.queue_chunk) + (counters ? "/c" : "");
       if (!c.batch_exec || c.batch_key != gkey) {
         if (c.batch_exec) HIPCHK(hipGraphExecDestroy(c.batch_exec));
         c.batch_exec = nullptr;
@@ -977,6 +989,196 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     HIPCHK(hipStreamSynchronize(c.stream));
     tm.d2h_ms = ms_since(td);
     read_counters(c, counters);
+    tm.total_ms = ms_since(t0);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
+    const optphot_sim_params* params, const uint64_t* photons, uint32_t n_sets,
+    uint64_t first_id, uint32_t time_bins, float time_max, const double* target,
+    const double* target_photons, double* hist, double* loss, double* grad,
+    const optphot_wavefront_tuning* forward_tuning, const optphot_wavefront_tuning* adjoint_tuning,
+    optphot_run_timing* timing) {
+  return guarded([&] {
+    if (!target || !target_photons || !hist || !loss || !grad)
+      throw InvalidArgument("chi2 step: NULL argument");
+    validate_batch(params, photons, n_sets, time_bins, time_max,
+                   reinterpret_cast<uint32_t*>(hist), OPTPHOT_BATCH_WAVEFRONT);
+    for (uint32_t e = 0; e < n_sets; ++e)
+      if (!(target_photons[e] > 0.0))
+        throw InvalidArgument("chi2 step: target_photons must be > 0");
+    auto merge = [](const optphot_wavefront_tuning* t, optphot_wavefront_tuning d) {
+      if (t) {
+        if (t->block_size) d.block_size = t->block_size;
+        if (t->steps_per_refill) d.steps_per_refill = t->steps_per_refill;
+        if (t->queue_chunk) d.queue_chunk = t->queue_chunk;
+        if (t->blocks_per_cu) d.blocks_per_cu = t->blocks_per_cu;
+      }
+      validate_block(d.block_size, "chi2 step");
+      if (d.queue_chunk < 64 || d.queue_chunk % 64 != 0 || d.steps_per_refill == 0)
+        throw InvalidArgument("chi2 step: queue_chunk must be a multiple of 64, "
+                              "steps_per_refill >= 1");
+      return d;
+    };
+    const optphot_wavefront_tuning tf = merge(forward_tuning, {256, 4, 256, 0});
+    const optphot_wavefront_tuning tb = merge(adjoint_tuning, {256, 2, 256, 0});
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+
+    std::vector<DevParams> table(n_sets);
+    std::vector<unsigned long long> offsets(n_sets + 1);
+    std::vector<double> inv_nt(n_sets);
+    offsets[0] = 0;
+    for (uint32_t e = 0; e < n_sets; ++e) {
+      table[e] = to_dev(params[e]);
+      offsets[e + 1] = offsets[e] + photons[e];
+      inv_nt[e] = 1.0 / target_photons[e];
+    }
+    const uint64_t total = offsets[n_sets];
+    const uint32_t words = 6u + time_bins;
+    const uint32_t bins1 = time_bins + 1u;
+    const size_t count_words = static_cast<size_t>(n_sets) * words;
+    const size_t zero_u64 = (count_words + 1) / 2;
+    const size_t wcount = static_cast<size_t>(n_sets) * bins1;
+
+    const auto ta = std::chrono::steady_clock::now();
+    bool grew = c.batch_table.ensure(table.size() * sizeof(DevParams));
+    grew |= c.batch_offsets.ensure(offsets.size() * 8);
+    grew |= c.batch_counts.ensure(zero_u64 * 8);
+    grew |= c.adj_weights.ensure(wcount * 8);
+    grew |= c.adj_grad.ensure(static_cast<size_t>(n_sets) * 5 * 8);
+    grew |= c.adj_queue.ensure((static_cast<size_t>(n_sets) + 1) * 8);
+    grew |= c.step_target.ensure(wcount * 8);
+    grew |= c.step_inv_nt.ensure(static_cast<size_t>(n_sets) * 8);
+    grew |= c.step_hist.ensure(wcount * 8);
+    grew |= c.step_loss.ensure(static_cast<size_t>(n_sets) * 8);
+    if (grew) {
+      tm.alloc_ms += ms_since(ta);
+      c.drop_graphs();
+    }
+    unsigned long long* scratch = ensure_scratch(c, 1, tm.alloc_ms);
+
+    const auto th = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(c.batch_table.ptr, table.data(), table.size() * sizeof(DevParams),
+                          hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.batch_offsets.ptr, offsets.data(), offsets.size() * 8,
+                          hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.step_target.ptr, target, wcount * 8, hipMemcpyHostToDevice,
+                          c.stream));
+    HIPCHK(hipMemcpyAsync(c.step_inv_nt.ptr, inv_nt.data(), inv_nt.size() * 8,
+                          hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.h2d_ms = ms_since(th);
+
+    // forward: the batched launch (as optphot_run_gpu_batch, wavefront method)
+    WavefrontShape sf{};
+    {
+      const int max_per_cu =
+          std::max(1, batch_blocks_per_cu(tf.block_size, batch_lds_bytes(n_sets, words)));
+      const int per_cu =
+          tf.blocks_per_cu > 0 ? std::min(tf.blocks_per_cu, max_per_cu) : max_per_cu;
+      const uint64_t needed = (total + tf.block_size - 1) / tf.block_size;
+      sf.block = tf.block_size;
+      sf.grid = static_cast<int>(std::max<uint64_t>(
+          1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
+      sf.steps_per_refill = tf.steps_per_refill;
+      sf.queue_chunk = tf.queue_chunk;
+    }
+    LaunchOut lo{};
+    lo.events = true;
+    lo.evt.counts = c.batch_counts.as<uint32_t>();
+    lo.evt.words = words;
+    lo.evt.time_bins = time_bins;
+    lo.evt.time_scale = static_cast<float>(time_bins) / time_max;  // transport.hpp:171
+    lo.evt.table = c.batch_table.as<DevParams>();
+    lo.evt.offsets = c.batch_offsets.as<unsigned long long>();
+    lo.evt.n_events = n_sets;
+    const PhiloxKey key = key_of(params[0]);
+    uint64_t max_chunks = 0;
+    for (uint32_t e = 0; e < n_sets; ++e)
+      max_chunks =
+          std::max<uint64_t>(max_chunks, (photons[e] + sf.queue_chunk - 1) / sf.queue_chunk);
+    const uint64_t n_slots = max_chunks * n_sets;
+
+    // adjoint shape and arguments
+    WavefrontShape sb{};
+    {
+      const int max_per_cu = std::max(1, adjoint_blocks_per_cu(tb.block_size));
+      const int per_cu =
+          tb.blocks_per_cu > 0 ? std::min(tb.blocks_per_cu, max_per_cu) : max_per_cu;
+      const uint64_t needed = (total + tb.block_size - 1) / tb.block_size;
+      sb.block = tb.block_size;
+      sb.grid = static_cast<int>(std::max<uint64_t>(
+          1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
+      sb.steps_per_refill = tb.steps_per_refill;
+      sb.queue_chunk = tb.queue_chunk;
+    }
+    AdjointArgs a{};
+    a.table = c.batch_table.as<DevParams>();
+    a.offsets = c.batch_offsets.as<unsigned long long>();
+    a.qheads = c.adj_queue.as<unsigned long long>();
+    a.closed = c.adj_queue.as<unsigned long long>() + n_sets;
+    a.n_events = n_sets;
+    a.time_bins = time_bins;
+    a.time_scale = lo.evt.time_scale;
+    a.weights = c.adj_weights.as<double>();
+    a.grad = c.adj_grad.as<double>();
+    int surface = static_cast<int>(params[0].surface);
+    for (uint32_t e = 1; e < n_sets; ++e)
+      if (static_cast<int>(params[e].surface) != surface) surface = -1;
+
+    // One graph: forward, chi-square weights, adjoint. The parameters, photon
+    // counts' offsets and targets are read from device buffers, so a replay
+    // with new values uploaded above is the same computation.
+    const std::string gkey =
+        std::to_string(n_sets) + "/" + std::to_string(n_slots) + "/" + std::to_string(first_id) +
+        "/" + std::to_string(params[0].seed) + "/" + std::to_string(words) + "/" +
+        std::to_string(lo.evt.time_scale) + "/" + std::to_string(surface) + "/f" +
+        std::to_string(sf.block) + "," + std::to_string(sf.grid) + "," +
+        std::to_string(sf.steps_per_refill) + "," + std::to_string(sf.queue_chunk) + "/b" +
+        std::to_string(sb.block) + "," + std::to_string(sb.grid) + "," +
+        std::to_string(sb.steps_per_refill) + "," + std::to_string(sb.queue_chunk);
+    if (!c.step_exec || c.step_key != gkey) {
+      if (c.step_exec) HIPCHK(hipGraphExecDestroy(c.step_exec));
+      c.step_exec = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      launch_zero_u64(scratch, kScratchSums, c.stream);  // queue head + counters
+      launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64, c.stream);
+      launch_batch_wavefront(key, first_id, n_slots, sf, scratch + kScratchQueue, lo, c.stream);
+      launch_chi2_weights(lo.evt.counts, words, bins1, a.offsets, c.step_target.as<double>(),
+                          c.step_inv_nt.as<double>(), c.step_hist.as<double>(),
+                          c.step_loss.as<double>(), c.adj_weights.as<double>(), n_sets,
+                          c.stream);
+      launch_zero_u64(a.qheads, static_cast<uint64_t>(n_sets) + 1, c.stream);
+      launch_zero_u64(reinterpret_cast<unsigned long long*>(a.grad),
+                      static_cast<uint64_t>(n_sets) * 5, c.stream);
+      launch_adjoint(table[0], surface, key, first_id, sb, a, c.stream, false);
+      HIPCHK(hipStreamEndCapture(c.stream, &g));
+      HIPCHK(hipGraphInstantiate(&c.step_exec, g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+      c.step_key = gkey;
+    }
+    HIPCHK(hipEventRecord(c.ev[0], c.stream));
+    HIPCHK(hipGraphLaunch(c.step_exec, c.stream));
+    HIPCHK(hipEventRecord(c.ev[1], c.stream));
+    tm.kernel_ms = c.elapsed(0, 1);
+    const auto td = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(hist, c.step_hist.ptr, wcount * 8, hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(loss, c.step_loss.ptr, static_cast<size_t>(n_sets) * 8,
+                          hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(grad, a.grad, static_cast<size_t>(n_sets) * 5 * 8,
+                          hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.d2h_ms = ms_since(td);
+    for (uint32_t e = 0; e < n_sets; ++e) {
+      const double inv = photons[e] ? 1.0 / static_cast<double>(photons[e]) : 0.0;
+      for (int j = 0; j < 5; ++j) grad[5 * e + j] *= inv;
+    }
     tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
   });
@@ -1020,11 +1222,9 @@ __attribute__((visibility("default"))) int optphot_run_gpu_adjoint(
     grew |= c.adj_weights.ensure(wcount * 8);
     grew |= c.adj_grad.ensure(static_cast<size_t>(n_events) * 5 * 8);
     grew |= c.adj_queue.ensure((static_cast<size_t>(n_events) + 1) * 8);
-    if (grew) tm.alloc_ms += ms_since(ta);
-    if (grew && c.batch_exec) {  // the batch graph refers to batch_table / batch_offsets
-      HIPCHK(hipGraphExecDestroy(c.batch_exec));
-      c.batch_exec = nullptr;
-      c.batch_key.clear();
+    if (grew) {
+      tm.alloc_ms += ms_since(ta);
+      c.drop_graphs();  // captured graphs refer to the old buffers
     }
     const auto th = std::chrono::steady_clock::now();
     HIPCHK(hipMemcpyAsync(c.batch_table.ptr, table.data(), table.size() * sizeof(DevParams),

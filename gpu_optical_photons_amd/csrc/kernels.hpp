@@ -126,9 +126,12 @@ int batch_blocks_per_cu(int block, size_t lds);
 // Adjoint (likelihood-ratio + pathwise) pass over the histories of a set of
 // parameter sets, recomputed from their Philox counters.
 // surface: the model shared by every set (0..3), or -1 if they differ. With
-// a single set its parameters P0 are passed as kernel arguments.
+// a single set and params_by_value its parameters P0 are passed as kernel
+// arguments; otherwise lanes read them from a.table (as a captured graph
+// that is replayed with new parameters must).
 void launch_adjoint(const DevParams& P0, int surface, PhiloxKey key, uint64_t first_id,
-                    const WavefrontShape& shape, const AdjointArgs& a, hipStream_t stream);
+                    const WavefrontShape& shape, const AdjointArgs& a, hipStream_t stream,
+                    bool params_by_value = true);
 int adjoint_blocks_per_cu(int block);
 
 // Sums the per-block histograms and sums into the flat u64 tally layout.
@@ -138,6 +141,15 @@ void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials
 
 // Fills n u64 words with zero (a kernel, so it may sit inside captured graphs).
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
+
+// Per-set chi-square loss of the batch tally's arrival-time histograms
+// (rows of `words` u32: 5 fates, then bins1 time bins) against target
+// fractions, and its gradient with respect to the histogram fractions
+// (weights for the adjoint). One block per set.
+void launch_chi2_weights(const uint32_t* counts, uint32_t words, uint32_t bins1,
+                         const unsigned long long* offsets, const double* target,
+                         const double* inv_nt, double* hist, double* loss, double* weights,
+                         uint32_t n_sets, hipStream_t stream);
 
 // VALU calibration kernel: `blocks` x 256 lanes, each iters x 8 FMAs (only odd
 // lanes when `half`); out holds blocks x 256 floats.

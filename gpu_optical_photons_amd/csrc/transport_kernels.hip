@@ -624,6 +624,54 @@ __global__ void zero_u64_kernel(unsigned long long* p, uint64_t n) {
     p[i] = 0ull;
 }
 
+// Chi-square loss of each parameter set's arrival-time histogram against its
+// target, and the loss gradient that weights the adjoint (one block per set).
+// h_k = counts_k / N (detected photons per bin, overflow included), f = sum h_k,
+// F = sum T_k; with 1/n_t the target's per-photon regulariser:
+//   loss = sum_k (h_k - T_k)^2 w_k + (f - F)^2 w_f,
+//   w_k = 1 / (T_k + 1/n_t),  w_f = 1 / (F (1 - F) + 1/n_t),
+//   dloss/dh_k = 2 (h_k - T_k) w_k + 2 (f - F) w_f.
+// All in FP64 (the cross-photon reduction).
+__global__ __launch_bounds__(256) void chi2_weights_kernel(
+    const uint32_t* counts, uint32_t words, uint32_t bins1, const unsigned long long* offsets,
+    const double* target, const double* inv_nt, double* hist, double* loss, double* weights) {
+  const uint32_t e = blockIdx.x;
+  const uint32_t* row = counts + static_cast<size_t>(e) * words + 5u;  // past the fate counts
+  const double n = static_cast<double>(offsets[e + 1] - offsets[e]);
+  const double* T = target + static_cast<size_t>(e) * bins1;
+  __shared__ double red_f[256], red_F[256], red_l[256];
+  double f = 0.0, F = 0.0, l = 0.0;
+  for (uint32_t k = threadIdx.x; k < bins1; k += blockDim.x) {
+    const double h = n > 0.0 ? static_cast<double>(row[k]) / n : 0.0;
+    hist[static_cast<size_t>(e) * bins1 + k] = h;
+    const double d = h - T[k];
+    f += h;
+    F += T[k];
+    l += d * d * (1.0 / (T[k] + inv_nt[e]));
+  }
+  red_f[threadIdx.x] = f;
+  red_F[threadIdx.x] = F;
+  red_l[threadIdx.x] = l;
+  __syncthreads();
+  for (uint32_t w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      red_f[threadIdx.x] += red_f[threadIdx.x + w];
+      red_F[threadIdx.x] += red_F[threadIdx.x + w];
+      red_l[threadIdx.x] += red_l[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  const double ft = red_f[0], Ft = red_F[0];
+  const double wf = 1.0 / (Ft * (1.0 - Ft) + inv_nt[e]);
+  const double df = ft - Ft;
+  if (threadIdx.x == 0) loss[e] = red_l[0] + df * df * wf;
+  for (uint32_t k = threadIdx.x; k < bins1; k += blockDim.x) {
+    const double h = hist[static_cast<size_t>(e) * bins1 + k];
+    weights[static_cast<size_t>(e) * bins1 + k] =
+        2.0 * (h - T[k]) * (1.0 / (T[k] + inv_nt[e])) + 2.0 * df * wf;
+  }
+}
+
 // Calibration of the VALU counters and issue rate: 8 independent FMA chains
 // per lane, `iters` x 8 v_fma_f32 per wave (the loop is unrolled by 4; its
 // control is scalar). With `half` only odd lanes run the loop, so a profiler's
@@ -763,6 +811,14 @@ void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials
                      dim3(threads), 0, stream, t.words, blocks, partials, sums, tally);
 }
 
+void launch_chi2_weights(const uint32_t* counts, uint32_t words, uint32_t bins1,
+                         const unsigned long long* offsets, const double* target,
+                         const double* inv_nt, double* hist, double* loss, double* weights,
+                         uint32_t n_sets, hipStream_t stream) {
+  hipLaunchKernelGGL(chi2_weights_kernel, dim3(n_sets), dim3(256), 0, stream, counts, words,
+                     bins1, offsets, target, inv_nt, hist, loss, weights);
+}
+
 void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
                        hipStream_t stream) {
   hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, stream, out, iters,
@@ -807,8 +863,9 @@ void launch_adjoint_surf(int surface, const DevParams& P0, PhiloxKey key, uint64
 }
 
 void launch_adjoint(const DevParams& P0, int surface, PhiloxKey key, uint64_t first_id,
-                    const WavefrontShape& s, const AdjointArgs& a, hipStream_t stream) {
-  if (a.n_events == 1)
+                    const WavefrontShape& s, const AdjointArgs& a, hipStream_t stream,
+                    bool params_by_value) {
+  if (a.n_events == 1 && params_by_value)
     launch_adjoint_surf<true>(surface, P0, key, first_id, s, a, stream);
   else
     launch_adjoint_surf<false>(surface, P0, key, first_id, s, a, stream);

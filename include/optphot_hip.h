@@ -219,6 +219,27 @@ int optphot_run_gpu_adjoint(const optphot_sim_params* params, const uint64_t* ph
                             float time_max, const double* weights, double* grad,
                             const optphot_wavefront_tuning* tuning, optphot_run_timing* timing);
 
+/* One calibration step for a batch of parameter sets, as a single captured
+ * hipGraph replayed while the shapes are unchanged: the batched forward
+ * launch (as optphot_run_gpu_batch), the per-set chi-square loss of the
+ * arrival-time histogram against target fractions target[e][k] (f64,
+ * [n_sets][time_bins + 1], overflow included),
+ *   loss_e = sum_k (h_ek - T_ek)^2 / (T_ek + 1/n_t) + (f_e - F_e)^2 / (F_e (1 - F_e) + 1/n_t),
+ * h_ek = detected photons of set e in bin k / photons[e], f_e = sum_k h_ek,
+ * F_e = sum_k T_ek, n_t = target_photons[e], and the adjoint pass with
+ * weights dloss_e/dh_ek. Writes hist (f64 [n_sets][time_bins + 1]), loss
+ * (f64 [n_sets]) and grad = dloss_e/dtheta_j (f64 [n_sets][5], theta as for
+ * optphot_run_gpu_adjoint). Parameters, offsets and targets are uploaded
+ * before each replay, so a replay with new parameters is the same
+ * computation as a fresh capture. Tunings may be NULL. */
+int optphot_run_gpu_chi2_step(const optphot_sim_params* params, const uint64_t* photons,
+                              uint32_t n_sets, uint64_t first_id, uint32_t time_bins,
+                              float time_max, const double* target, const double* target_photons,
+                              double* hist, double* loss, double* grad,
+                              const optphot_wavefront_tuning* forward_tuning,
+                              const optphot_wavefront_tuning* adjoint_tuning,
+                              optphot_run_timing* timing);
+
 /* Calibration of profiler VALU counters: launches `blocks` x 256 lanes, each
  * running iters x 8 independent v_fma_f32 (only the odd lanes of every wave
  * when half_lanes != 0), and returns the kernel time. A wave executes

@@ -100,3 +100,35 @@ def test_gradient_agrees_with_finite_differences(backend):
             assert lr[j] == 0.0
         else:
             assert abs(lr[j] - fd[j]) <= 4.0 * se, name
+
+
+@pytest.mark.parametrize("n_sets", [1, 6])
+def test_chi2_step_graph_equals_separate_calls(backend, n_sets):
+    """The one-graph calibration step (forward, chi-square weights on the
+    device, adjoint) against the same quantities from separate calls, over
+    three replays with new parameters (the graph must read them afresh)."""
+    from gpu_optical_photons_amd import adjoint
+    from gpu_optical_photons_amd.batch import EventBatch
+    from gpu_optical_photons_amd.calibrate import c5_template
+    bins, tmax, n, nt = 96, 80.0, 150_000, 400_000.0
+    tpl = c5_template(3)
+    rng = np.random.default_rng(n_sets)
+    base = np.array([2000.0, 300.0, 0.97, 1.5, 1.63])
+    targets = EventBatch([adjoint.with_theta(c5_template(99), base * (1 + 0.05 * rng.normal(size=5)))
+                          for _ in range(n_sets)], np.full(n_sets, int(nt), np.uint64))
+    T, _ = adjoint.histograms(targets, bins, tmax)
+    for rep in range(3):
+        th = base * (1 + 0.04 * rng.normal(size=(n_sets, 5)))
+        batch = EventBatch([adjoint.with_theta(tpl, r) for r in th],
+                           np.full(n_sets, n, np.uint64))
+        print(f"[chi2 step] sets {n_sets} replay {rep}", flush=True)
+        hist, loss, grad, _ = adjoint.run_gpu_chi2_step(batch, T, nt, bins, tmax)
+        h, _ = adjoint.histograms(batch, bins, tmax)
+        assert np.array_equal(hist, h)
+        np.testing.assert_allclose(loss, adjoint.chi2_loss(h, T, n, nt), rtol=1e-12)
+        f, F = h.sum(1, keepdims=True), T.sum(1, keepdims=True)
+        wk = 1.0 / (T + 1.0 / nt)
+        wf = 1.0 / (F * (1.0 - F) + 1.0 / nt)
+        dl_dh = 2.0 * (h - T) * wk + 2.0 * (f - F) * wf
+        g, _ = adjoint.run_gpu_adjoint(batch, dl_dh, bins, tmax)
+        np.testing.assert_allclose(grad, g, rtol=1e-9, atol=1e-12 * np.abs(g).max())
