@@ -1,8 +1,10 @@
 """Autotuner for the launch shape of the persistent wavefront kernels.
 
-Tunes block size, steps between refills ("wavefront length"), the queue
-chunk a wave claims per atomic, and persistent blocks per CU, by timing the
-real kernel on a sample of the workload. Results never depend on the choice
+Tunes block size, the refill policy -- steps between refills ("wavefront
+length"), or the successor kernel's emission threshold (lanes lacking a
+successor before a wave emits) --, the queue chunk a wave claims per atomic,
+and persistent blocks per CU, by timing the real kernel on a sample of the
+workload. Results never depend on the choice
 (integer tallies, per-photon streams; checked by the sweep itself and by the
 GPU tests), only the speed does. Choices persist in tuning.json next to this
 file, keyed by a workload signature (kernel, surface model, source, which bulk
@@ -21,10 +23,12 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 
-DEFAULT = {"block_size": 256, "steps_per_refill": 4, "queue_chunk": 256, "blocks_per_cu": 0}
+DEFAULT = {"block_size": 256, "steps_per_refill": 4, "queue_chunk": 256, "blocks_per_cu": 0,
+           "successor_threshold": 0}
 
 BLOCKS = (64, 128, 256, 512, 1024)
 STEPS = (1, 2, 4, 8, 16)
+THRESHOLDS = (16, 32, 48, 64)
 CHUNKS = (128, 512, 2048)
 
 
@@ -48,12 +52,12 @@ def _load():
 def choice_for(kernel: str, p, c=None) -> dict:
     entry = _load().get(signature(kernel, p, c))
     if entry:
-        return dict(entry["tuning"])
+        return dict(DEFAULT) | dict(entry["tuning"])
     return dict(DEFAULT)
 
 
-def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS, reps: int = 2,
-          save: bool = True, log=print) -> dict:
+def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
+          thresholds=THRESHOLDS, reps: int = 2, save: bool = True, log=print) -> dict:
     """Times every candidate on n photons; keeps the fastest. Every candidate
     must give the same tally word for word."""
     import numpy as np
@@ -64,8 +68,13 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS, 
         raise ValueError(f"unknown kernel {kernel!r}")
     ref = None
     results = []
-    for b, k, ch in itertools.product(blocks, steps, chunks):
-        tu = {"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0}
+    candidates = [{"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0,
+                   "successor_threshold": 0}
+                  for b, k, ch in itertools.product(blocks, steps, chunks)]
+    candidates += [{"block_size": b, "steps_per_refill": 1, "queue_chunk": ch, "blocks_per_cu": 0,
+                    "successor_threshold": th}
+                   for b, th, ch in itertools.product(blocks, thresholds, chunks)]
+    for tu in candidates:
         best = math.inf
         for _ in range(reps):
             t, tm, st, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)
@@ -77,7 +86,10 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS, 
             raise AssertionError(f"tally differs for tuning {tu}: the kernel is not "
                                  "schedule-independent")
         results.append((best, tu))
-        log(f"  {kernel} block={b:4d} steps={k:2d} chunk={ch:5d}: {best:8.3f} ms")
+        policy = (f"successor>={tu['successor_threshold']:2d}" if tu["successor_threshold"]
+                  else f"steps={tu['steps_per_refill']:2d}")
+        log(f"  {kernel} block={tu['block_size']:4d} {policy:14s} chunk={tu['queue_chunk']:5d}: "
+            f"{best:8.3f} ms")
     results.sort(key=lambda r: r[0])
     best_ms, best_tu = results[0]
     entry = {"tuning": best_tu, "kernel_ms": best_ms, "photons": n,

@@ -59,7 +59,8 @@ class StepCounters(ctypes.Structure):
 
 class WavefrontTuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int), ("steps_per_refill", ctypes.c_uint32),
-                ("queue_chunk", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_int)]
+                ("queue_chunk", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_int),
+                ("successor_threshold", ctypes.c_uint32)]
 
 
 class _RecordsC(ctypes.Structure):

@@ -408,19 +408,23 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
 
 WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, bool records,
                                bool tally, size_t lds, uint64_t n) {
-  optphot_wavefront_tuning t{256, 4, 256, 0};
+  optphot_wavefront_tuning t{256, 4, 256, 0, 0};
   if (tu) {
     if (tu->block_size) t.block_size = tu->block_size;
     if (tu->steps_per_refill) t.steps_per_refill = tu->steps_per_refill;
     if (tu->queue_chunk) t.queue_chunk = tu->queue_chunk;
     if (tu->blocks_per_cu) t.blocks_per_cu = tu->blocks_per_cu;
+    if (tally && !records) t.successor_threshold = tu->successor_threshold;
   }
+  if (t.successor_threshold > 64)
+    throw InvalidArgument("wavefront: successor_threshold must be in [0, 64]");
   validate_block(t.block_size, "wavefront");
   if (t.queue_chunk < 64 || t.queue_chunk % 64 != 0 || t.queue_chunk > (1u << 24))
     throw InvalidArgument("wavefront: queue_chunk must be a multiple of 64 in [64, 2^24]");
   if (t.steps_per_refill == 0)
     throw InvalidArgument("wavefront: steps_per_refill must be >= 1");
-  const int max_per_cu = std::max(1, wavefront_blocks_per_cu(records, tally, t.block_size, lds));
+  const int max_per_cu = std::max(
+      1, wavefront_blocks_per_cu(records, tally, t.block_size, lds, t.successor_threshold > 0));
   const int per_cu = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, max_per_cu) : max_per_cu;
   const uint64_t needed = (n + t.block_size - 1) / t.block_size;
   WavefrontShape s{};
@@ -429,6 +433,7 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
       1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
   s.steps_per_refill = t.steps_per_refill;
   s.queue_chunk = t.queue_chunk;
+  s.successor_threshold = t.successor_threshold;
   return s;
 }
 

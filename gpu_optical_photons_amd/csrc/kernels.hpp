@@ -89,6 +89,9 @@ struct AdjointArgs {
 struct WavefrontShape {
   int block, grid;
   uint32_t steps_per_refill, queue_chunk;
+  // Tally launches: > 0 selects the kernel whose lanes keep an emitted
+  // successor photon, emitting when this many lanes of a wave lack one.
+  uint32_t successor_threshold = 0;
 };
 
 // History-based launches: one lane per photon (records), or a grid-stride
@@ -158,6 +161,7 @@ void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
 
 // Occupancy queries (blocks per CU for a block size and dynamic LDS bytes).
 int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);
-int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds_bytes);
+int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds,
+                            bool successor = false);
 
 }  // namespace ophip

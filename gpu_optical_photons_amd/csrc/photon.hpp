@@ -230,7 +230,12 @@ __device__ __forceinline__ V3 rotate_about(V3 n, float cos_theta, float phi_turn
 __device__ __forceinline__ float rayleigh_cos(float u) {
   const float q = 4.0f * u - 2.0f;
   const float s = f_sqrt(q * q + 1.0f);
-  const float a = (q >= 0.0f) ? p_cbrt(q + s) : 1.0f / p_cbrt(s - q);
+  // A = cbrt(q + s), or 1 / cbrt(s - q) for q < 0: one cube root on the
+  // selected argument (a wave with lanes on both sides would otherwise run
+  // two Newton iterations side by side), then the lane's own reciprocal.
+  const bool pos = q >= 0.0f;
+  const float c = p_cbrt(pos ? q + s : s - q);
+  const float a = pos ? c : 1.0f / c;
   return __builtin_fminf(1.0f, __builtin_fmaxf(-1.0f, a - 1.0f / a));
 }
 

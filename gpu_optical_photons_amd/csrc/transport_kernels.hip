@@ -298,6 +298,137 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
   flush_block<kTally, kCount>(out, lds, snb, sns, wc);
 }
 
+// ---------------------------------------------------------------------------
+// Persistent wavefront kernel with successors (tally). Besides its current
+// photon every lane may hold an already emitted successor (position,
+// direction, emission time and Philox window). Emission runs for the lanes of
+// a wave that lack a successor only once at least `threshold` of them do (or
+// when nothing else is left), so the emission code runs with many active
+// lanes; a lane whose photon finishes swaps its successor in at once and
+// keeps stepping. Per photon the computation is the one of the other kernels.
+template <bool kCount, int kSurf>
+__global__ __launch_bounds__(1024) void wavefront_next_kernel(DevParams P0, PhiloxKey key,
+                                                              uint64_t first_id, uint64_t n,
+                                                              uint32_t threshold, uint32_t chunk,
+                                                              unsigned long long* queue,
+                                                              LaunchOut out) {
+  const DevParams& P = P0;
+  extern __shared__ uint32_t lds[];
+  zero_lds(lds, out.tally.words);
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  WaveCounters<kCount> wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;
+
+  uint64_t q_next = 0, q_end = 0;  // photons claimed by this wave, not yet emitted
+  bool q_done = false;             // the global queue is exhausted
+  bool alive = false, has_next = false;
+  uint32_t done = 0, fate = kMaxSteps;
+  Stream rng;
+  Photon ph;
+  // the successor: what emission produced (path and counts start at zero;
+  // the Philox window holds blocks 0 and 1, so the next block is 2)
+  V3 nx_p{}, nx_d{};
+  float nx_t = 0.0f;
+  uint32_t nw0 = 0, nw1 = 0, nw2 = 0, nw3 = 0, nw4 = 0, nw5 = 0, nw6 = 0, nw7 = 0;
+  uint32_t npos = 0, nid_lo = 0, nid_hi = 0;
+
+  for (;;) {
+    // ---- emit successors for the lanes that lack one ------------------------
+    // Emit when `threshold` lanes lack a successor, when a quarter of that
+    // many lanes have neither photon nor successor, or when no lane is alive.
+    const uint64_t need = __ballot(!has_next);
+    const uint32_t n_need = popc(need);
+    const uint32_t n_idle = popc(__ballot(!alive && !has_next));
+    const bool go = n_need >= threshold || 4u * n_idle >= threshold || __ballot(alive) == 0;
+    if (!q_done && n_need != 0 && go) {
+      const uint64_t have = q_end - q_next;
+      const uint32_t t1 = have < n_need ? static_cast<uint32_t>(have) : n_need;
+      const uint64_t b1 = q_next;
+      q_next += t1;
+      uint64_t b2 = 0;
+      uint32_t t2 = 0;
+      if (t1 < n_need) {
+        unsigned long long got = 0;
+        if (lane == 0) got = atomicAdd(queue, static_cast<unsigned long long>(chunk));
+        got = first_lane_u64(got);
+        if (got >= n) {
+          q_done = true;
+        } else {
+          q_end = (got + chunk < n) ? got + chunk : n;
+          const uint64_t avail = q_end - got;
+          t2 = (n_need - t1 < avail) ? n_need - t1 : static_cast<uint32_t>(avail);
+          b2 = got;
+          q_next = got + t2;
+        }
+      }
+      const uint32_t r = rank_in(need);
+      if (!has_next && r < t1 + t2) {
+        const uint64_t idx = r < t1 ? b1 + r : b2 + (r - t1);
+        Stream g;
+        Photon e;
+        g.start(first_id + idx, key, 0u);
+        emit(P, g, e);
+        nx_p = e.p;
+        nx_d = e.d;
+        nx_t = e.t;
+        nw0 = g.w0, nw1 = g.w1, nw2 = g.w2, nw3 = g.w3;
+        nw4 = g.w4, nw5 = g.w5, nw6 = g.w6, nw7 = g.w7;
+        npos = g.pos, nid_lo = g.id_lo, nid_hi = g.id_hi;
+        has_next = true;
+      }
+      wc.add(kCntEmitted, t1 + t2);
+    }
+    // ---- lanes without a photon take their successor ------------------------
+    if (!alive && has_next) {
+      ph.p = nx_p;
+      ph.d = nx_d;
+      ph.t = nx_t;
+      ph.path = 0.0f;
+      ph.nb = 0u;
+      ph.ns = 0u;
+      rng.w0 = nw0, rng.w1 = nw1, rng.w2 = nw2, rng.w3 = nw3;
+      rng.w4 = nw4, rng.w5 = nw5, rng.w6 = nw6, rng.w7 = nw7;
+      rng.pos = npos;
+      rng.blk = 2u;
+      rng.id_lo = nid_lo;
+      rng.id_hi = nid_hi;
+      has_next = false;
+      done = 0;
+      fate = kMaxSteps;
+      alive = P.max_steps > 0u;
+      if (!alive) {  // max_steps == 0: finished at emission
+        finish<false, true>(P, out, lds, 0, ph, fate, snb, sns);
+      }
+    }
+    const uint64_t act = __ballot(alive);
+    if (act == 0) {
+      if (q_done && __ballot(has_next) == 0) break;
+      continue;
+    }
+
+    // ---- one transport step ----------------------------------------------------
+    StepInfo info{};
+    bool finished = false;
+    if (alive) {
+      rng.refill(key);
+      const bool still = step<kSurf>(P, rng, ph, fate, info);
+      ++done;
+      if (!still || done >= P.max_steps) {
+        if (still) fate = kMaxSteps;
+        finished = true;
+        alive = false;
+      }
+    }
+    wc.add_step(act, info);
+    if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+    if (finished) finish<false, true>(P, out, lds, 0, ph, fate, snb, sns);
+  }
+  __syncthreads();
+  flush_block<true, kCount>(out, lds, snb, sns, wc);
+}
+
 // Column sums of the [blocks][words] partial histograms (and the [blocks][2]
 // sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
 // every gridDim.y-th block row, so enough waves are in flight to stream the
@@ -705,6 +836,15 @@ void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
                      first_id, n, out);
 }
 
+template <bool C, int S>
+void launch_wavefront_next_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                             const WavefrontShape& s, unsigned long long* queue,
+                             const LaunchOut& out, hipStream_t stream) {
+  const size_t lds = static_cast<size_t>(out.tally.words) * sizeof(uint32_t);
+  hipLaunchKernelGGL((wavefront_next_kernel<C, S>), dim3(s.grid), dim3(s.block), lds, stream, P,
+                     key, first_id, n, s.successor_threshold, s.queue_chunk, queue, out);
+}
+
 template <bool R, bool T, bool C, int S = -1>
 void launch_wavefront_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                         const WavefrontShape& s, unsigned long long* queue, const LaunchOut& out,
@@ -719,6 +859,23 @@ template <bool C>
 void launch_wavefront_tally(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                             const WavefrontShape& s, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream) {
+  if (s.successor_threshold > 0) {
+    switch (P.surface) {
+      case kPolished:
+        launch_wavefront_next_t<C, kPolished>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      case kBlack:
+        launch_wavefront_next_t<C, kBlack>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      case kSpecular:
+        launch_wavefront_next_t<C, kSpecular>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      default:
+        launch_wavefront_next_t<C, kLambertian>(P, key, first_id, n, s, queue, out, stream);
+        break;
+    }
+    return;
+  }
   switch (P.surface) {
     case kPolished:
       launch_wavefront_t<false, true, C, kPolished>(P, key, first_id, n, s, queue, out,
@@ -838,8 +995,9 @@ int history_blocks_per_cu(bool tally, int block, size_t lds) {
                : blocks_per_cu(history_kernel<true, false, false, false>, block, lds);
 }
 
-int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds) {
+int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds, bool successor) {
   if (records) return blocks_per_cu(wavefront_kernel<true, false, true>, block, lds);
+  if (successor) return blocks_per_cu(wavefront_next_kernel<false, kLambertian>, block, lds);
   return blocks_per_cu(wavefront_kernel<false, true, false, kLambertian>, block, lds);
 }
 

@@ -112,6 +112,11 @@ typedef struct optphot_wavefront_tuning {
   uint32_t steps_per_refill;  /* transport steps between work-queue refills */
   uint32_t queue_chunk;       /* photons a wave claims per queue atomic */
   int blocks_per_cu;          /* persistent blocks per compute unit */
+  /* Wavefront tally only: 0 = refill finished lanes every steps_per_refill
+   * steps; n in [1, 64] = every lane keeps an emitted successor photon and a
+   * wave emits successors once n of its lanes lack one (steps_per_refill is
+   * then unused). Results are identical either way. */
+  uint32_t successor_threshold;
 } optphot_wavefront_tuning;
 
 /* ---- Reference GPU backend entry points ----------------------------------- */
