@@ -24,11 +24,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 
 DEFAULT = {"block_size": 256, "steps_per_refill": 4, "queue_chunk": 256, "blocks_per_cu": 0,
-           "successor_threshold": 0}
+           "successor_threshold": 0, "stream_in_lds": 0}
 
 BLOCKS = (64, 128, 256, 512, 1024)
 STEPS = (1, 2, 4, 8, 16)
 THRESHOLDS = (16, 32, 48, 64)
+RING_BLOCKS = (128, 256, 512)
+RING_STEPS = (1, 2, 3, 4)
 CHUNKS = (128, 512, 2048)
 
 
@@ -57,7 +59,8 @@ def choice_for(kernel: str, p, c=None) -> dict:
 
 
 def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
-          thresholds=THRESHOLDS, reps: int = 2, save: bool = True, log=print) -> dict:
+          thresholds=THRESHOLDS, ring_blocks=RING_BLOCKS, ring_steps=RING_STEPS, reps: int = 2,
+          save: bool = True, log=print) -> dict:
     """Times every candidate on n photons; keeps the fastest. Every candidate
     must give the same tally word for word."""
     import numpy as np
@@ -69,16 +72,23 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
     ref = None
     results = []
     candidates = [{"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0,
-                   "successor_threshold": 0}
+                   "successor_threshold": 0, "stream_in_lds": 0}
                   for b, k, ch in itertools.product(blocks, steps, chunks)]
     candidates += [{"block_size": b, "steps_per_refill": 1, "queue_chunk": ch, "blocks_per_cu": 0,
-                    "successor_threshold": th}
+                    "successor_threshold": th, "stream_in_lds": 0}
                    for b, th, ch in itertools.product(blocks, thresholds, chunks)]
+    candidates += [{"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0,
+                    "successor_threshold": 0, "stream_in_lds": 1}
+                   for b, k, ch in itertools.product(ring_blocks, ring_steps, chunks)]
     for tu in candidates:
         best = math.inf
-        for _ in range(reps):
-            t, tm, st, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)
-            best = min(best, tm.kernel_ms)
+        try:
+            for _ in range(reps):
+                t, tm, st, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)
+                best = min(best, tm.kernel_ms)
+        except ValueError as e:  # a shape this tally does not fit (LDS)
+            log(f"  skipped {tu}: {e}")
+            continue
         flat = t.flat()
         if ref is None:
             ref = flat
@@ -86,7 +96,8 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
             raise AssertionError(f"tally differs for tuning {tu}: the kernel is not "
                                  "schedule-independent")
         results.append((best, tu))
-        policy = (f"successor>={tu['successor_threshold']:2d}" if tu["successor_threshold"]
+        policy = (f"ring steps={tu['steps_per_refill']:2d}" if tu["stream_in_lds"]
+                  else f"successor>={tu['successor_threshold']:2d}" if tu["successor_threshold"]
                   else f"steps={tu['steps_per_refill']:2d}")
         log(f"  {kernel} block={tu['block_size']:4d} {policy:14s} chunk={tu['queue_chunk']:5d}: "
             f"{best:8.3f} ms")

@@ -408,23 +408,30 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
 
 WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, bool records,
                                bool tally, size_t lds, uint64_t n) {
-  optphot_wavefront_tuning t{256, 4, 256, 0, 0};
+  optphot_wavefront_tuning t{256, 4, 256, 0, 0, 0};
   if (tu) {
     if (tu->block_size) t.block_size = tu->block_size;
     if (tu->steps_per_refill) t.steps_per_refill = tu->steps_per_refill;
     if (tu->queue_chunk) t.queue_chunk = tu->queue_chunk;
     if (tu->blocks_per_cu) t.blocks_per_cu = tu->blocks_per_cu;
-    if (tally && !records) t.successor_threshold = tu->successor_threshold;
+    if (tally && !records) {
+      t.successor_threshold = tu->successor_threshold;
+      t.stream_in_lds = tu->stream_in_lds;
+    }
   }
   if (t.successor_threshold > 64)
     throw InvalidArgument("wavefront: successor_threshold must be in [0, 64]");
+  if (t.stream_in_lds && ring_lds_bytes(static_cast<uint32_t>(lds / 4), t.block_size) > c.max_lds)
+    throw InvalidArgument("wavefront: tally plus LDS streams exceed the LDS of a block; use a "
+                          "smaller block_size");
   validate_block(t.block_size, "wavefront");
   if (t.queue_chunk < 64 || t.queue_chunk % 64 != 0 || t.queue_chunk > (1u << 24))
     throw InvalidArgument("wavefront: queue_chunk must be a multiple of 64 in [64, 2^24]");
   if (t.steps_per_refill == 0)
     throw InvalidArgument("wavefront: steps_per_refill must be >= 1");
-  const int max_per_cu = std::max(
-      1, wavefront_blocks_per_cu(records, tally, t.block_size, lds, t.successor_threshold > 0));
+  const int max_per_cu =
+      std::max(1, wavefront_blocks_per_cu(records, tally, t.block_size, lds,
+                                          t.successor_threshold > 0, t.stream_in_lds != 0));
   const int per_cu = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, max_per_cu) : max_per_cu;
   const uint64_t needed = (n + t.block_size - 1) / t.block_size;
   WavefrontShape s{};
@@ -434,6 +441,7 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   s.steps_per_refill = t.steps_per_refill;
   s.queue_chunk = t.queue_chunk;
   s.successor_threshold = t.successor_threshold;
+  s.stream_in_lds = t.stream_in_lds;
   return s;
 }
 

@@ -92,6 +92,9 @@ struct WavefrontShape {
   // Tally launches: > 0 selects the kernel whose lanes keep an emitted
   // successor photon, emitting when this many lanes of a wave lack one.
   uint32_t successor_threshold = 0;
+  // Tally launches: keep each lane's Philox window in an LDS ring (takes
+  // precedence over successor_threshold).
+  uint32_t stream_in_lds = 0;
 };
 
 // History-based launches: one lane per photon (records), or a grid-stride
@@ -161,7 +164,10 @@ void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
 
 // Occupancy queries (blocks per CU for a block size and dynamic LDS bytes).
 int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);
+// LDS bytes of a wavefront tally launch with the streams in LDS: the tally
+// words, then 16 ring words per lane.
+size_t ring_lds_bytes(uint32_t tally_words, int block);
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds,
-                            bool successor = false);
+                            bool successor = false, bool ring = false);
 
 }  // namespace ophip

@@ -144,6 +144,86 @@ struct Stream {
   __device__ __forceinline__ uint32_t draws() const { return 4u * (blk - 2u) + pos; }
 };
 
+// The same stream with its window in LDS: a ring of 16 words per lane,
+// slot-major (slot j of lane l is ring_base[j * stride + l], stride = the
+// block size), so a wave's 64 accesses to one slot index fall in distinct
+// banks. `pos` counts the words consumed and `wr` the words generated (4 per
+// Philox block); words pos .. wr-1 are readable and wr - pos <= 16. Reading
+// a word is one LDS load at a computed address instead of a select chain
+// over registers, and a lane with room can take a block whenever the wave
+// runs a generation pass, so passes are shared out over many steps.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+struct RingStream {
+  lds_u32* ring;  // this lane's slot 0
+  uint32_t stride;
+  uint32_t pos, wr;
+  uint32_t id_lo, id_hi;
+
+  static constexpr uint32_t kSlots = 16u;
+
+  __device__ __forceinline__ void reset(uint64_t photon_id) {
+    id_lo = static_cast<uint32_t>(photon_id);
+    id_hi = static_cast<uint32_t>(photon_id >> 32);
+    pos = 0u;
+    wr = 0u;
+  }
+  __device__ __forceinline__ uint32_t level() const { return wr - pos; }
+  __device__ __forceinline__ bool room() const { return wr - pos <= kSlots - 4u; }
+  // Appends the next Philox block (counter wr / 4) to the ring.
+  __device__ __forceinline__ void generate(PhiloxKey key) {
+    uint32_t a, b, c, d;
+    philox10(wr >> 2, id_lo, id_hi, key, a, b, c, d);
+    const uint32_t sl = wr & (kSlots - 1u);
+    ring[sl * stride] = a;
+    ring[(sl + 1u) * stride] = b;
+    ring[(sl + 2u) * stride] = c;
+    ring[(sl + 3u) * stride] = d;
+    wr += 4u;
+  }
+  // Word pos + K (the caller keeps pos + K < wr).
+  template <int K>
+  __device__ __forceinline__ uint32_t peek() const {
+    return ring[((pos + K) & (kSlots - 1u)) * stride];
+  }
+};
+
+// Correctly rounded a / b and sqrt(x) for operands in a normal range, where
+// the compiler's full IEEE sequences spend instructions on range scaling and
+// special-case fix-up that cannot act:
+//  * div_n: the standard refined-reciprocal sequence (rcp, one Newton step on
+//    the reciprocal, two residual corrections of the quotient) without
+//    v_div_scale / v_div_fixup. Those only rescale or replace the result when
+//    an operand is 0, inf, NaN or denormal, the exponents differ by >= 96, or
+//    the quotient or 1 / b is denormal; for b normal with 2^-100 < |b| < 2^100
+//    and a == 0 or 2^-100 < |a| < 2^100 (quotient normal) both give the same
+//    bits (a == 0 gives +-0 either way).
+//  * sqrt_n: hardware estimate plus the one-ulp correction of both
+//    neighbours, without the 2^32 pre-scaling of inputs below 2^-96 and the
+//    +-0 / +inf pass-through: identical for x == +-0 and x >= 2^-96 finite.
+// Every call site states why its operands are in range.
+__device__ __forceinline__ float div_n(float a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  const float e = __builtin_fmaf(-b, r, 1.0f);
+  r = __builtin_fmaf(e, r, r);
+  float q = a * r;
+  float t = __builtin_fmaf(-b, q, a);
+  q = __builtin_fmaf(t, r, q);
+  t = __builtin_fmaf(-b, q, a);
+  return __builtin_fmaf(t, r, q);
+}
+
+__device__ __forceinline__ float sqrt_n(float x) {
+  float y = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(y) - 1u);
+  const float up = __uint_as_float(__float_as_uint(y) + 1u);
+  const float r_dn = __builtin_fmaf(-dn, y, x);
+  const float r_up = __builtin_fmaf(-up, y, x);
+  y = (r_dn <= 0.0f) ? dn : y;
+  y = (r_up > 0.0f) ? up : y;
+  return y;
+}
+
 // ---------------------------------------------------------------------------
 // Transcendentals from correctly rounded operations only (the reference's
 // portable math, pmath.hpp:62-130): bit-identical to the CPU build with
@@ -157,7 +237,8 @@ __device__ __forceinline__ float p_log(float x) {
     m = 0.5f * m;
     ++e;
   }
-  const float s = (m - 1.0f) / (m + 1.0f);
+  // m in [0.707, 1.415): m - 1 is 0 or at least 2^-24 in magnitude, m + 1 in [1.7, 2.5]
+  const float s = div_n(m - 1.0f, m + 1.0f);
   const float s2 = s * s;
   const float poly =
       1.0f + s2 * (1.0f / 3.0f +
@@ -192,14 +273,16 @@ __device__ __forceinline__ void p_sincos_turns(float u, float& sn, float& cs) {
   cs = (quadrant == 1 || quadrant == 2) ? -b : b;
 }
 
+// Called for x in [1, 4.3] only (Rayleigh sampling), where every Newton
+// quotient is normal: y ~ x^(1/3) in [0.9, 1.7], 3 y^2 in [2.4, 8.7], and
+// y^3 - x is 0 or at least 2^-24 in magnitude.
 __device__ __forceinline__ float p_cbrt(float x) {
   float y = __uint_as_float(__float_as_uint(x) / 3u + 0x2a514067u);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) y = y - (y * y * y - x) / (3.0f * y * y);
+  for (int i = 0; i < 4; ++i) y = y - div_n(y * y * y - x, 3.0f * y * y);
   return y;
 }
 
-__device__ __forceinline__ float f_sqrt(float x) { return __builtin_sqrtf(x); }
 
 // ---------------------------------------------------------------------------
 // Vector helpers (vec3.hpp): only what the step needs, same operation order.
@@ -212,31 +295,34 @@ struct V3 {
 // normalize((st cp) t1 + (st sp) t2 + ct n) with normalize(a) = (1/|a|) a.
 __device__ __forceinline__ V3 rotate_about(V3 n, float cos_theta, float phi_turns) {
   const float sign = __builtin_copysignf(1.0f, n.z);
-  const float a = -1.0f / (sign + n.z);
+  // n is a unit vector and sign = copysign(1, n.z): sign + n.z in [1, 2] or [-2, -1]
+  const float a = div_n(-1.0f, sign + n.z);
   const float b = n.x * n.y * a;
   const V3 t1{1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
   const V3 t2{b, sign + n.y * n.y * a, -n.y};
-  const float st = f_sqrt(__builtin_fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
+  // |cos| <= 1: 1 - cos^2 is 0 or at least 2^-24
+  const float st = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
   float sp, cp;
   p_sincos_turns(phi_turns, sp, cp);
   const float c1 = st * cp, c2 = st * sp;
   const V3 d{(c1 * t1.x + c2 * t2.x) + cos_theta * n.x, (c1 * t1.y + c2 * t2.y) + cos_theta * n.y,
              (c1 * t1.z + c2 * t2.z) + cos_theta * n.z};
-  const float inv = 1.0f / f_sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+  // |d|^2 = 1 up to rounding
+  const float inv = div_n(1.0f, sqrt_n(d.x * d.x + d.y * d.y + d.z * d.z));
   return V3{inv * d.x, inv * d.y, inv * d.z};
 }
 
 // Cardano root of the Rayleigh CDF (sampling.hpp:39-44).
 __device__ __forceinline__ float rayleigh_cos(float u) {
   const float q = 4.0f * u - 2.0f;
-  const float s = f_sqrt(q * q + 1.0f);
+  const float s = sqrt_n(q * q + 1.0f);  // q in [-2, 2]: q^2 + 1 in [1, 5]
   // A = cbrt(q + s), or 1 / cbrt(s - q) for q < 0: one cube root on the
   // selected argument (a wave with lanes on both sides would otherwise run
   // two Newton iterations side by side), then the lane's own reciprocal.
   const bool pos = q >= 0.0f;
   const float c = p_cbrt(pos ? q + s : s - q);
-  const float a = pos ? c : 1.0f / c;
-  return __builtin_fminf(1.0f, __builtin_fmaxf(-1.0f, a - 1.0f / a));
+  const float a = pos ? c : div_n(1.0f, c);  // c in [1, 1.62], so a in [0.61, 1.62]
+  return __builtin_fminf(1.0f, __builtin_fmaxf(-1.0f, a - div_n(1.0f, a)));
 }
 
 // Tally binning (transport.hpp:170-181); `time_scale` = time_bins / time_max
@@ -345,13 +431,15 @@ struct StepInfo {
 };
 
 // Emission (transport.hpp:55-67): [x, y, z if volume], cos, phi, time.
-// The stream window must hold 8 fresh words (pos == 0 after start()).
-__device__ __forceinline__ void emit(const DevParams& P, Stream& rng, Photon& ph) {
+// The stream must hold the photon's first 6 words (Stream: pos == 0 after
+// start(); RingStream: level() >= 6).
+template <class RNG>
+__device__ __forceinline__ void emit(const DevParams& P, RNG& rng, Photon& ph) {
   uint32_t o = 0;
   if (P.source == kVolume) {
-    const float u1 = to_unit(rng.peek<0>());
-    const float u2 = to_unit(rng.peek<1>());
-    const float u3 = to_unit(rng.peek<2>());
+    const float u1 = to_unit(rng.template peek<0>());
+    const float u2 = to_unit(rng.template peek<1>());
+    const float u3 = to_unit(rng.template peek<2>());
     ph.p = V3{P.hx * (2.0f * u1 - 1.0f), P.hy * (2.0f * u2 - 1.0f), P.hz * (2.0f * u3 - 1.0f)};
     o = 3;
   } else {
@@ -360,12 +448,12 @@ __device__ __forceinline__ void emit(const DevParams& P, Stream& rng, Photon& ph
   // The window holds words 0..7 and o <= 3, so the offsets o..o+2 are read
   // through peek with pos advanced first.
   rng.pos += o;
-  const float ua = to_unit(rng.peek<0>());
-  const float ub = to_unit(rng.peek<1>());
-  const float ut = to_unit(rng.peek<2>());
+  const float ua = to_unit(rng.template peek<0>());
+  const float ub = to_unit(rng.template peek<1>());
+  const float ut = to_unit(rng.template peek<2>());
   rng.pos += 3u;
   const float cos_t = 2.0f * ua - 1.0f;
-  const float sin_t = f_sqrt(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));
+  const float sin_t = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));  // 0 or >= 2^-24
   float sp, cp;
   p_sincos_turns(ub, sp, cp);
   ph.d = V3{sin_t * cp, sin_t * sp, cos_t};
@@ -376,17 +464,17 @@ __device__ __forceinline__ void emit(const DevParams& P, Stream& rng, Photon& ph
 }
 
 // One transport step (transport.hpp:71-145). Returns true while the photon
-// is alive; otherwise *fate is set. The caller must have called
-// rng.refill() so that pos <= 3.
+// is alive; otherwise *fate is set. The stream must hold the step's (up to 5)
+// words: Stream after refill() (pos <= 3), RingStream with level() >= 5.
 // kSurf >= 0 fixes the surface model at compile time (a launch whose events
 // all share it); kSurf < 0 reads it from P.
-template <int kSurf = -1, bool kAdj = false>
-__device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph, uint32_t& fate,
+template <int kSurf = -1, bool kAdj = false, class RNG>
+__device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, uint32_t& fate,
                                      StepInfo& info, AdjState* adj = nullptr) {
   const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
   // --- two exponential distances, always drawn (transport.hpp:76-77) -------
-  const float u_abs = to_unit(rng.peek<0>());
-  const float u_sca = to_unit(rng.peek<1>());
+  const float u_abs = to_unit(rng.template peek<0>());
+  const float u_sca = to_unit(rng.template peek<1>());
   float s_abs = kInf, s_sca = kInf;
   if (P.abs_length != kInf) s_abs = -P.abs_length * p_log(u_abs);
   if (P.scat_length != kInf) s_sca = -P.scat_length * p_log(u_sca);
@@ -461,7 +549,9 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
     const float eta2 = readout ? P.eta2_det : P.eta2_out;
     const bool matched = readout ? P.det_matched : P.out_matched;
     const float sin2_t = eta2 * __builtin_fmaxf(0.0f, 1.0f - cos_i * cos_i);
-    const float cos_t = f_sqrt(1.0f - sin2_t);
+    // 1 - sin2_t is in (0, 1] and at least 2^-24 unless total internal
+    // reflection (sin2_t >= 1), where R is not used
+    const float cos_t = sqrt_n(1.0f - sin2_t);
     const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
     const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
     const float R = 0.5f * (rs * rs + rp * rp);
@@ -473,7 +563,7 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
   }
   // --- the surface decision draw (u <= prob survives, transport.hpp:41) -----
   const bool decide = readout || polished || refl;
-  const bool survived = decide && to_unit(rng.peek<2>()) <= prob;
+  const bool survived = decide && to_unit(rng.template peek<2>()) <= prob;
   const bool lambert_reflect = refl && lambert && survived;
   if (kAdj && scored && (survived || !readout)) {
     // Bernoulli score: d log R (reflected) or d log (1 - R) (transmitted);
@@ -499,14 +589,14 @@ __device__ __forceinline__ bool step(const DevParams& P, Stream& rng, Photon& ph
   if (scatter || lambert_reflect) {
     // Rayleigh: about the direction, cos from Cardano, draws 3, 4.
     // Lambertian: about the inward normal -n, cos = sqrt(u), draws 4, 5.
-    const uint32_t wc = scatter ? rng.peek<2>() : rng.peek<3>();
-    const uint32_t wp = scatter ? rng.peek<3>() : rng.peek<4>();
+    const uint32_t wc = scatter ? rng.template peek<2>() : rng.template peek<3>();
+    const uint32_t wp = scatter ? rng.template peek<3>() : rng.template peek<4>();
     const float uc = to_unit(wc), up = to_unit(wp);
     float cos_theta;
     if (scatter)
       cos_theta = rayleigh_cos(uc);
     else
-      cos_theta = f_sqrt(uc);
+      cos_theta = sqrt_n(uc);  // uc in [2^-24, 1]
     const float s = plus ? 1.0f : -1.0f;
     // -n keeps the reference's signed zeros: -(+0) = -0 off-axis.
     const V3 inward{axis == 0 ? -s : -0.0f, axis == 1 ? -s : -0.0f, axis == 2 ? -s : -0.0f};

@@ -429,6 +429,132 @@ __global__ __launch_bounds__(1024) void wavefront_next_kernel(DevParams P0, Phil
   flush_block<true, kCount>(out, lds, snb, sns, wc);
 }
 
+// ---------------------------------------------------------------------------
+// Persistent wavefront kernel with the photons' random streams in LDS (tally).
+// Each lane's Philox window is a 16-word ring in LDS (RingStream). Before a
+// step the wave runs generation passes until every lane about to draw holds
+// enough words (5 for a step, plus 3 or 6 for an emission); in each pass
+// every such lane with room appends a block, so the lanes stay topped up and
+// most steps need one pass or none. Emission draws from the same ring, so a
+// refill costs no Philox code of its own. Finished photons are tallied at the
+// next refill point, once per `steps_per_refill` steps instead of every step.
+// Per photon the words drawn and the arithmetic are those of the other
+// kernels.
+template <bool kCount, int kSurf>
+__global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, PhiloxKey key,
+                                                              uint64_t first_id, uint64_t n,
+                                                              uint32_t steps_per_refill,
+                                                              uint32_t chunk,
+                                                              unsigned long long* queue,
+                                                              LaunchOut out) {
+  const DevParams& P = P0;
+  extern __shared__ uint32_t lds[];
+  const uint32_t ring_off = (out.tally.words + 3u) & ~3u;
+  zero_lds(lds, out.tally.words);
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  WaveCounters<kCount> wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;
+
+  uint64_t q_next = 0, q_end = 0;  // photons claimed by this wave, not yet started
+  bool q_done = false;             // the global queue is exhausted
+  bool alive = false;    // a photon in transport
+  bool fresh = false;    // an id assigned, emission pending
+  bool pending = false;  // a finished photon not yet tallied
+  uint32_t done = 0, fate = kMaxSteps;
+  RingStream rng;
+  rng.ring = (lds_u32*)(lds + ring_off) + threadIdx.x;
+  rng.stride = blockDim.x;
+  rng.pos = 0u;
+  rng.wr = 0u;
+  rng.id_lo = rng.id_hi = 0u;
+  Photon ph;
+  const uint32_t emit_words = P.source == kVolume ? 6u : 3u;
+
+  for (uint32_t it = 0;; ++it) {
+    // ---- refill point: tally the finished photons, assign ids to free lanes --
+    const bool all_dead = __ballot(alive) == 0;
+    if (all_dead || it % steps_per_refill == 0) {
+      if (pending) {
+        finish<false, true>(P, out, lds, 0, ph, fate, snb, sns);
+        pending = false;
+      }
+      const uint64_t dead = __ballot(!alive);
+      if (dead != 0 && !q_done) {
+        const uint32_t need = popc(dead);
+        const uint64_t have = q_end - q_next;
+        const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+        const uint64_t b1 = q_next;
+        q_next += t1;
+        uint64_t b2 = 0;
+        uint32_t t2 = 0;
+        if (t1 < need) {
+          unsigned long long got = 0;
+          if (lane == 0) got = atomicAdd(queue, static_cast<unsigned long long>(chunk));
+          got = first_lane_u64(got);
+          if (got >= n) {
+            q_done = true;
+          } else {
+            q_end = (got + chunk < n) ? got + chunk : n;
+            const uint64_t avail = q_end - got;
+            t2 = (need - t1 < avail) ? need - t1 : static_cast<uint32_t>(avail);
+            b2 = got;
+            q_next = got + t2;
+          }
+        }
+        const uint32_t r = rank_in(dead);
+        if (!alive && r < t1 + t2) {
+          rng.reset(first_id + (r < t1 ? b1 + r : b2 + (r - t1)));
+          fresh = true;
+        }
+        wc.add(kCntEmitted, t1 + t2);
+      }
+      if (all_dead && __ballot(fresh) == 0) {
+        if (q_done) break;
+        continue;
+      }
+    }
+
+    // ---- generation passes -------------------------------------------------
+    const bool drawing = alive || fresh;
+    const uint32_t need = fresh ? emit_words + 5u : 5u;
+    while (__ballot(drawing && rng.level() < need) != 0) {
+      if (drawing && rng.room()) rng.generate(key);
+    }
+
+    // ---- emission of the newly assigned photons ----------------------------
+    if (fresh) {
+      emit(P, rng, ph);
+      fresh = false;
+      done = 0;
+      fate = kMaxSteps;
+      alive = P.max_steps > 0u;
+      if (!alive) pending = true;  // max_steps == 0: finished at emission
+    }
+
+    // ---- one transport step --------------------------------------------------
+    const uint64_t act = __ballot(alive);
+    if (act == 0) continue;
+    StepInfo info{};
+    bool finished = false;
+    if (alive) {
+      const bool still = step<kSurf>(P, rng, ph, fate, info);
+      ++done;
+      if (!still || done >= P.max_steps) {
+        if (still) fate = kMaxSteps;
+        finished = true;
+        alive = false;
+      }
+    }
+    wc.add_step(act, info);
+    if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+    if (finished) pending = true;
+  }
+  __syncthreads();
+  flush_block<true, kCount>(out, lds, snb, sns, wc);
+}
+
 // Column sums of the [blocks][words] partial histograms (and the [blocks][2]
 // sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
 // every gridDim.y-th block row, so enough waves are in flight to stream the
@@ -837,6 +963,15 @@ void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
 }
 
 template <bool C, int S>
+void launch_wavefront_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                             const WavefrontShape& s, unsigned long long* queue,
+                             const LaunchOut& out, hipStream_t stream) {
+  hipLaunchKernelGGL((wavefront_ring_kernel<C, S>), dim3(s.grid), dim3(s.block),
+                     ring_lds_bytes(out.tally.words, s.block), stream, P, key, first_id, n,
+                     s.steps_per_refill, s.queue_chunk, queue, out);
+}
+
+template <bool C, int S>
 void launch_wavefront_next_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                              const WavefrontShape& s, unsigned long long* queue,
                              const LaunchOut& out, hipStream_t stream) {
@@ -859,6 +994,23 @@ template <bool C>
 void launch_wavefront_tally(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                             const WavefrontShape& s, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream) {
+  if (s.stream_in_lds) {
+    switch (P.surface) {
+      case kPolished:
+        launch_wavefront_ring_t<C, kPolished>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      case kBlack:
+        launch_wavefront_ring_t<C, kBlack>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      case kSpecular:
+        launch_wavefront_ring_t<C, kSpecular>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      default:
+        launch_wavefront_ring_t<C, kLambertian>(P, key, first_id, n, s, queue, out, stream);
+        break;
+    }
+    return;
+  }
   if (s.successor_threshold > 0) {
     switch (P.surface) {
       case kPolished:
@@ -903,6 +1055,11 @@ int blocks_per_cu(K kernel, int block, size_t lds) {
 }
 
 }  // namespace
+
+size_t ring_lds_bytes(uint32_t tally_words, int block) {
+  return (((static_cast<size_t>(tally_words) + 3u) & ~size_t{3}) +
+          static_cast<size_t>(RingStream::kSlots) * block) * sizeof(uint32_t);
+}
 
 // Instantiated variants: records (no counters), tally with / without
 // counters, and the plain per-event launches of a batch.
@@ -995,8 +1152,12 @@ int history_blocks_per_cu(bool tally, int block, size_t lds) {
                : blocks_per_cu(history_kernel<true, false, false, false>, block, lds);
 }
 
-int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds, bool successor) {
+int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds, bool successor,
+                            bool ring) {
   if (records) return blocks_per_cu(wavefront_kernel<true, false, true>, block, lds);
+  if (ring)
+    return blocks_per_cu(wavefront_ring_kernel<false, kLambertian>, block,
+                         ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
   if (successor) return blocks_per_cu(wavefront_next_kernel<false, kLambertian>, block, lds);
   return blocks_per_cu(wavefront_kernel<false, true, false, kLambertian>, block, lds);
 }

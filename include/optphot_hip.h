@@ -117,6 +117,10 @@ typedef struct optphot_wavefront_tuning {
    * wave emits successors once n of its lanes lack one (steps_per_refill is
    * then unused). Results are identical either way. */
   uint32_t successor_threshold;
+  /* Wavefront tally only: nonzero keeps each lane's Philox window in an LDS
+   * ring (generation passes shared across steps; takes precedence over
+   * successor_threshold). Results are identical either way. */
+  uint32_t stream_in_lds;
 } optphot_wavefront_tuning;
 
 /* ---- Reference GPU backend entry points ----------------------------------- */

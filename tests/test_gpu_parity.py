@@ -123,7 +123,13 @@ def test_tally_equals_oracle(backend, oracle_portable, name):
         assert t.boundary_sum == ref["boundary_sum"]
         assert t.scatter_sum == ref["scatter_sum"]
     for tuning in (None, dict(block_size=64, steps_per_refill=1, queue_chunk=64),
-                   dict(block_size=1024, steps_per_refill=16, queue_chunk=4096)):
+                   dict(block_size=1024, steps_per_refill=16, queue_chunk=4096),
+                   # successor-photon kernel
+                   dict(block_size=256, queue_chunk=512, successor_threshold=32),
+                   dict(block_size=64, queue_chunk=64, successor_threshold=1),
+                   # Philox windows in an LDS ring, with deferred tallying
+                   dict(block_size=512, steps_per_refill=2, queue_chunk=512, stream_in_lds=1),
+                   dict(block_size=64, steps_per_refill=5, queue_chunk=64, stream_in_lds=1)):
         t, tm, st, cnt = backend.run_gpu_tally_wavefront(p, c, n, tuning=tuning)
         np.testing.assert_array_equal(t.flat(), backend.Tally.from_flat(
             np.concatenate([ref["fate_counts"], ref["time_hist"], ref["xy_hist"].ravel(),
