@@ -249,6 +249,9 @@ def main():
     c = TallyConfig(256, 100.0, 64)
     n = args.photons
     tuning = autotune.choice_for("wavefront_tally", p, c)
+    kernel_name = ("wavefront_ring_kernel" if tuning.get("stream_in_lds")
+                   else "wavefront_next_kernel" if tuning.get("successor_threshold")
+                   else "wavefront_kernel")
 
     for _ in range(args.warmup):
         backend.run_gpu_tally_wavefront(p, c, n, tuning=tuning, counters=False)
@@ -317,7 +320,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / roofline.FP32_VECTOR_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "wavefront_kernel<records=false, tally=true>",
+            "kernel": kernel_name,
             "kernel_ms": transport_avg,
             "kernel_ms_note": "transport kernel + tally reduction, HIP events on the library stream",
             "algorithmic_ops_per_launch": ops,
@@ -326,7 +329,7 @@ def main():
         },
     }
     tr = load_traffic()
-    if tr:
+    if tr and tr.get("kernel", "wavefront_kernel") == kernel_name:
         line["roofline"]["traffic"] = tr.get("hbm_bytes_per_launch")
         line["roofline"]["traffic_source"] = tr.get("source")
     if not args.only_main:
