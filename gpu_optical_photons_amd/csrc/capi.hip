@@ -881,12 +881,13 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     optphot_run_timing* timing) {
   return guarded([&] {
     validate_batch(params, photons, n_events, time_bins, time_max, event_tally, method);
-    optphot_wavefront_tuning tu{256, 4, 256, 0};
+    optphot_wavefront_tuning tu{256, 2, 256, 0, 0, 0};
     if (tuning) {
       if (tuning->block_size) tu.block_size = tuning->block_size;
       if (tuning->steps_per_refill) tu.steps_per_refill = tuning->steps_per_refill;
       if (tuning->queue_chunk) tu.queue_chunk = tuning->queue_chunk;
       if (tuning->blocks_per_cu) tu.blocks_per_cu = tuning->blocks_per_cu;
+      tu.stream_in_lds = tuning->stream_in_lds;
     }
     validate_block(tu.block_size, "batch");
     Context& c = ctx();
@@ -916,9 +917,13 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
       tm.alloc_ms += ms_since(ta);
       c.drop_graphs();  // buffers moved: captured graphs are stale
     }
+    const bool ring = tu.stream_in_lds != 0 && method == OPTPHOT_BATCH_WAVEFRONT;
+    if (ring && batch_ring_lds_bytes(batch_lds_bytes(n_events, words), tu.block_size) > c.max_lds)
+      throw InvalidArgument("batch: event tallies plus LDS streams exceed the LDS of a block; use "
+                            "a smaller block_size");
     const WavefrontShape s = [&] {
-      const int max_per_cu =
-          std::max(1, batch_blocks_per_cu(tu.block_size, batch_lds_bytes(n_events, words)));
+      const int max_per_cu = std::max(
+          1, batch_blocks_per_cu(tu.block_size, batch_lds_bytes(n_events, words), ring));
       const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
       const uint64_t needed = (total + tu.block_size - 1) / tu.block_size;
       WavefrontShape w{};
@@ -927,6 +932,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
           1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
       w.steps_per_refill = tu.steps_per_refill;
       w.queue_chunk = tu.queue_chunk;
+      w.stream_in_lds = ring ? 1u : 0u;
       return w;
     }();
     if (s.queue_chunk < 64 || s.queue_chunk % 64 != 0 || s.steps_per_refill == 0)
@@ -964,7 +970,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
                                "/" + std::to_string(words) + "/" + std::to_string(lo.evt.time_scale) +
                                "/" + std::to_string(s.block) + "/" + std::to_string(s.grid) + "/" +
                                std::to_string(s.steps_per_refill) + "/" +
-                               std::to_string(s.queue_chunk) + (counters ? "/c" : "");
+                               std::to_string(s.queue_chunk) + (counters ? "/c" : "") +
+                               (s.stream_in_lds ? "/ring" : "");
       if (!c.batch_exec || c.batch_key != gkey) {
         if (c.batch_exec) HIPCHK(hipGraphExecDestroy(c.batch_exec));
         c.batch_exec = nullptr;
@@ -1027,6 +1034,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
         if (t->steps_per_refill) d.steps_per_refill = t->steps_per_refill;
         if (t->queue_chunk) d.queue_chunk = t->queue_chunk;
         if (t->blocks_per_cu) d.blocks_per_cu = t->blocks_per_cu;
+        d.stream_in_lds = t->stream_in_lds;
       }
       validate_block(d.block_size, "chi2 step");
       if (d.queue_chunk < 64 || d.queue_chunk % 64 != 0 || d.steps_per_refill == 0)
@@ -1034,8 +1042,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
                               "steps_per_refill >= 1");
       return d;
     };
-    const optphot_wavefront_tuning tf = merge(forward_tuning, {256, 4, 256, 0});
-    const optphot_wavefront_tuning tb = merge(adjoint_tuning, {256, 2, 256, 0});
+    const optphot_wavefront_tuning tf = merge(forward_tuning, {256, 2, 256, 0, 0, 0});
+    const optphot_wavefront_tuning tb = merge(adjoint_tuning, {256, 2, 256, 0, 0, 0});
     Context& c = ctx();
     std::lock_guard<std::mutex> lock(c.mu);
     c.init();
@@ -1090,8 +1098,12 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
     // forward: the batched launch (as optphot_run_gpu_batch, wavefront method)
     WavefrontShape sf{};
     {
-      const int max_per_cu =
-          std::max(1, batch_blocks_per_cu(tf.block_size, batch_lds_bytes(n_sets, words)));
+      const bool ring = tf.stream_in_lds != 0;
+      if (ring && batch_ring_lds_bytes(batch_lds_bytes(n_sets, words), tf.block_size) > c.max_lds)
+        throw InvalidArgument("chi2 step: event tallies plus LDS streams exceed the LDS of a "
+                              "block; use a smaller block_size");
+      const int max_per_cu = std::max(
+          1, batch_blocks_per_cu(tf.block_size, batch_lds_bytes(n_sets, words), ring));
       const int per_cu =
           tf.blocks_per_cu > 0 ? std::min(tf.blocks_per_cu, max_per_cu) : max_per_cu;
       const uint64_t needed = (total + tf.block_size - 1) / tf.block_size;
@@ -1100,6 +1112,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
           1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
       sf.steps_per_refill = tf.steps_per_refill;
       sf.queue_chunk = tf.queue_chunk;
+      sf.stream_in_lds = ring ? 1u : 0u;
     }
     LaunchOut lo{};
     lo.events = true;
@@ -1152,7 +1165,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
         "/" + std::to_string(params[0].seed) + "/" + std::to_string(words) + "/" +
         std::to_string(lo.evt.time_scale) + "/" + std::to_string(surface) + "/f" +
         std::to_string(sf.block) + "," + std::to_string(sf.grid) + "," +
-        std::to_string(sf.steps_per_refill) + "," + std::to_string(sf.queue_chunk) + "/b" +
+        std::to_string(sf.steps_per_refill) + "," + std::to_string(sf.queue_chunk) +
+        (sf.stream_in_lds ? ",ring" : "") + "/b" +
         std::to_string(sb.block) + "," + std::to_string(sb.grid) + "," +
         std::to_string(sb.steps_per_refill) + "," + std::to_string(sb.queue_chunk);
     if (!c.step_exec || c.step_key != gkey) {
@@ -1205,7 +1219,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_adjoint(
     if (!weights || !grad) throw InvalidArgument("adjoint: NULL weights / grad");
     validate_batch(params, photons, n_events, time_bins, time_max,
                    reinterpret_cast<uint32_t*>(grad), OPTPHOT_BATCH_WAVEFRONT);
-    optphot_wavefront_tuning tu{256, 2, 256, 0};
+    optphot_wavefront_tuning tu{256, 2, 256, 0, 0, 0};
     if (tuning) {
       if (tuning->block_size) tu.block_size = tuning->block_size;
       if (tuning->steps_per_refill) tu.steps_per_refill = tuning->steps_per_refill;

@@ -127,7 +127,10 @@ void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, 
 // kBatchLdsMax bytes; batch_lds_bytes() is that size, or 0 for global atomics.
 constexpr size_t kBatchLdsMax = 32768;
 size_t batch_lds_bytes(uint32_t n_events, uint32_t words);
-int batch_blocks_per_cu(int block, size_t lds);
+// LDS bytes of a batch launch with ring streams: the event tallies (if any),
+// then 16 ring words per lane.
+size_t batch_ring_lds_bytes(size_t tally_lds, int block);
+int batch_blocks_per_cu(int block, size_t lds, bool ring = false);
 
 // Adjoint (likelihood-ratio + pathwise) pass over the histories of a set of
 // parameter sets, recomputed from their Philox counters.

@@ -57,6 +57,12 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 
 __device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
 
+// Batched events: queue slots a wave claims per atomic. Slot c is chunk c / E
+// of event c mod E, so a wave's range spans kSlotsPerClaim different events;
+// many slots are empty (events with fewer chunks than the largest), and those
+// cost the wave a compare instead of a contended global atomic.
+constexpr uint32_t kSlotsPerClaim = 16;
+
 // Wave-uniform branch-event counters (kept in SGPRs). Compiled in only for
 // the launches that report them: 16 live 64-bit scalars otherwise push the
 // kernel's scalar registers into VGPR lanes.
@@ -555,6 +561,146 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
   flush_block<true, kCount>(out, lds, snb, sns, wc);
 }
 
+// ---------------------------------------------------------------------------
+// Batched events with the photons' random streams in LDS rings (see
+// wavefront_ring_kernel): the slot queue and per-event parameters of
+// batch_kernel, the ring streams, generation passes and deferred tallying of
+// wavefront_ring_kernel. With kLds the block's event tallies come first in
+// LDS, the rings after them.
+template <bool kCount, bool kLds>
+__global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_t first_id,
+                                                          uint32_t steps_per_refill,
+                                                          uint32_t chunk, uint64_t n_slots,
+                                                          unsigned long long* queue,
+                                                          LaunchOut out) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lds_words = kLds ? out.evt.n_events * out.evt.words : 0u;
+  if (kLds) {
+    zero_lds(lds, lds_words);
+    __syncthreads();
+  }
+  LaunchOut loc = out;
+  if (kLds) loc.evt.counts = lds;
+  const uint32_t lane = lane_id();
+  const uint32_t E = out.evt.n_events;
+  const unsigned long long* off = out.evt.offsets;
+  WaveCounters<kCount> wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
+
+  uint32_t we = 0;                 // event of the wave's claimed range (uniform)
+  uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
+  uint64_t s_next = 0, s_end = 0;  // slots claimed by the wave, not yet opened
+  bool q_done = false;
+  bool alive = false, fresh = false, pending = false;
+  uint32_t pset = 0;
+  uint32_t done = 0, fate = kMaxSteps;
+  DevParams P{};
+  RingStream rng;
+  rng.ring = (lds_u32*)(lds + ((lds_words + 3u) & ~3u)) + threadIdx.x;
+  rng.stride = blockDim.x;
+  rng.pos = 0u;
+  rng.wr = 0u;
+  rng.id_lo = rng.id_hi = 0u;
+  Photon ph;
+
+  for (uint32_t it = 0;; ++it) {
+    const bool all_dead = __ballot(alive) == 0;
+    if (all_dead || it % steps_per_refill == 0) {
+      if (pending) {
+        finish<false, false, true>(P, loc, nullptr, 0, ph, fate, snb, sns, pset);
+        pending = false;
+      }
+      const uint64_t dead = __ballot(!alive);
+      if (dead != 0 && !q_done) {
+        const uint32_t need = popc(dead);
+        const uint64_t have = q_end - q_next;
+        const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+        const uint64_t b1 = q_next;
+        const uint32_t s1 = we;
+        q_next += t1;
+        uint64_t b2 = 0;
+        uint32_t t2 = 0;
+        while (t1 < need && !q_done) {  // uniform: take the next non-empty slot
+          if (s_next >= s_end) {  // the wave's slot range is used up: claim the next one
+            unsigned long long c = 0;
+            if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(kSlotsPerClaim));
+            c = first_lane_u64(c);
+            if (c >= n_slots) {
+              q_done = true;
+              break;
+            }
+            s_next = c;
+            s_end = (c + kSlotsPerClaim < n_slots) ? c + kSlotsPerClaim : n_slots;
+          }
+          const uint64_t c = s_next++;
+          const uint32_t e = static_cast<uint32_t>(c % E);
+          const uint64_t start = (c / E) * chunk;
+          const uint64_t ne = off[e + 1] - off[e];
+          if (start >= ne) continue;
+          we = e;
+          q_next = start;
+          q_end = (start + chunk < ne) ? start + chunk : ne;
+          t2 = (need - t1 < q_end - q_next) ? need - t1 : static_cast<uint32_t>(q_end - q_next);
+          b2 = q_next;
+          q_next += t2;
+          break;  // one chunk per refill
+        }
+        const uint32_t r = rank_in(dead);
+        if (!alive && r < t1 + t2) {
+          pset = r < t1 ? s1 : we;
+          const uint64_t idx = r < t1 ? b1 + r : b2 + (r - t1);
+          P = out.evt.table[pset];
+          rng.reset(first_id + off[pset] + idx);
+          fresh = true;
+        }
+        wc.add(kCntEmitted, t1 + t2);
+      }
+      if (all_dead && __ballot(fresh) == 0) {
+        if (q_done) break;
+        continue;
+      }
+    }
+
+    const bool drawing = alive || fresh;
+    const uint32_t need = fresh ? (P.source == kVolume ? 6u : 3u) + 5u : 5u;
+    while (__ballot(drawing && rng.level() < need) != 0) {
+      if (drawing && rng.room()) rng.generate(key);
+    }
+    if (fresh) {
+      emit(P, rng, ph);
+      fresh = false;
+      done = 0;
+      fate = kMaxSteps;
+      alive = P.max_steps > 0u;
+      if (!alive) pending = true;
+    }
+
+    const uint64_t act = __ballot(alive);
+    if (act == 0) continue;
+    StepInfo info{};
+    bool finished = false;
+    if (alive) {
+      const bool still = step(P, rng, ph, fate, info);
+      ++done;
+      if (!still || done >= P.max_steps) {
+        if (still) fate = kMaxSteps;
+        finished = true;
+        alive = false;
+      }
+    }
+    wc.add_step(act, info);
+    if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+    if (finished) pending = true;
+  }
+  wc.flush(out.counters);
+  if (kLds) {
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < lds_words; w += blockDim.x)
+      if (lds[w]) atomicAdd(out.evt.counts + w, lds[w]);
+  }
+}
+
 // Column sums of the [blocks][words] partial histograms (and the [blocks][2]
 // sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
 // every gridDim.y-th block row, so enough waves are in flight to stream the
@@ -592,6 +738,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
 
   uint32_t we = 0;                 // event of the wave's claimed range (uniform)
   uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
+  uint64_t s_next = 0, s_end = 0;  // slots claimed by the wave, not yet opened
   bool q_done = false;
   bool alive = false;
   uint32_t pset = 0;
@@ -612,14 +759,19 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
       q_next += t1;
       uint64_t b2 = 0;
       uint32_t t2 = 0;
-      while (t1 < need && !q_done) {  // uniform: claim the next non-empty slot
-        unsigned long long c = 0;
-        if (lane == 0) c = atomicAdd(queue, 1ull);
-        c = first_lane_u64(c);
-        if (c >= n_slots) {
-          q_done = true;
-          break;
+      while (t1 < need && !q_done) {  // uniform: take the next non-empty slot
+        if (s_next >= s_end) {  // the wave's slot range is used up: claim the next one
+          unsigned long long c = 0;
+          if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(kSlotsPerClaim));
+          c = first_lane_u64(c);
+          if (c >= n_slots) {
+            q_done = true;
+            break;
+          }
+          s_next = c;
+          s_end = (c + kSlotsPerClaim < n_slots) ? c + kSlotsPerClaim : n_slots;
         }
+        const uint64_t c = s_next++;
         const uint32_t e = static_cast<uint32_t>(c % E);
         const uint64_t start = (c / E) * chunk;
         const uint64_t ne = off[e + 1] - off[e];
@@ -1094,6 +1246,26 @@ void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
                             const WavefrontShape& s, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream) {
   const size_t lds = batch_lds_bytes(out.evt.n_events, out.evt.words);
+  if (s.stream_in_lds) {
+    const size_t bytes = batch_ring_lds_bytes(lds, s.block);
+    if (lds && out.counters)
+      hipLaunchKernelGGL((batch_ring_kernel<true, true>), dim3(s.grid), dim3(s.block), bytes,
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
+                         out);
+    else if (lds)
+      hipLaunchKernelGGL((batch_ring_kernel<false, true>), dim3(s.grid), dim3(s.block), bytes,
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
+                         out);
+    else if (out.counters)
+      hipLaunchKernelGGL((batch_ring_kernel<true, false>), dim3(s.grid), dim3(s.block), bytes,
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
+                         out);
+    else
+      hipLaunchKernelGGL((batch_ring_kernel<false, false>), dim3(s.grid), dim3(s.block), bytes,
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
+                         out);
+    return;
+  }
   if (lds && out.counters)
     hipLaunchKernelGGL((batch_kernel<true, true>), dim3(s.grid), dim3(s.block), lds, stream, key,
                        first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
@@ -1197,7 +1369,17 @@ size_t batch_lds_bytes(uint32_t n_events, uint32_t words) {
   return b <= kBatchLdsMax ? b : 0;
 }
 
-int batch_blocks_per_cu(int block, size_t lds) {
+size_t batch_ring_lds_bytes(size_t tally_lds, int block) {
+  return ((tally_lds + 15u) & ~size_t{15}) +
+         static_cast<size_t>(RingStream::kSlots) * block * sizeof(uint32_t);
+}
+
+int batch_blocks_per_cu(int block, size_t lds, bool ring) {
+  if (ring)
+    return lds ? blocks_per_cu(batch_ring_kernel<false, true>, block,
+                               batch_ring_lds_bytes(lds, block))
+               : blocks_per_cu(batch_ring_kernel<false, false>, block,
+                               batch_ring_lds_bytes(0, block));
   return lds ? blocks_per_cu(batch_kernel<false, true>, block, lds)
              : blocks_per_cu(batch_kernel<false, false>, block, 0);
 }

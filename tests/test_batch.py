@@ -46,6 +46,14 @@ def test_small_batch_matches_oracle_event_by_event(backend, oracle_portable):
     w, _, _ = run_gpu_batch(b, TB, TMAX, method=WAVEFRONT)
     p, _, _ = run_gpu_batch(b, TB, TMAX, method=PLAIN)
     assert np.array_equal(w, p)
+    r, _, _ = run_gpu_batch(b, TB, TMAX, method=WAVEFRONT, tuning=dict(stream_in_lds=1))
+    assert np.array_equal(w, r)
+    # few events: per-block event tallies in LDS (with and without ring streams)
+    few = make_events(8, seed=12, photons_per_mev=3000)
+    fw, _, _ = run_gpu_batch(few, TB, TMAX, method=WAVEFRONT)
+    fr, _, _ = run_gpu_batch(few, TB, TMAX, method=WAVEFRONT, tuning=dict(stream_in_lds=1))
+    fp, _, _ = run_gpu_batch(few, TB, TMAX, method=PLAIN)
+    assert np.array_equal(fw, fp) and np.array_equal(fr, fp)
     off = b.offsets
     for e in range(b.n_events):
         assert np.array_equal(w[e].astype(np.int64), _oracle_row(oracle_portable, b.params[e],
@@ -78,7 +86,9 @@ def test_batch_first_id_and_tuning_invariance(backend):
     b = make_events(200, seed=3, photons_per_mev=2000)
     ref, _, _ = run_gpu_batch(b, TB, TMAX, first_id=12345)
     for tu in (dict(block_size=64, steps_per_refill=1, queue_chunk=64),
-               dict(block_size=1024, steps_per_refill=8, queue_chunk=2048)):
+               dict(block_size=1024, steps_per_refill=8, queue_chunk=2048),
+               dict(block_size=512, steps_per_refill=2, queue_chunk=256, stream_in_lds=1),
+               dict(block_size=64, steps_per_refill=3, queue_chunk=64, stream_in_lds=1)):
         w, _, _ = run_gpu_batch(b, TB, TMAX, first_id=12345, method=WAVEFRONT, tuning=tu)
         assert np.array_equal(w, ref)
     other, _, _ = run_gpu_batch(b, TB, TMAX, first_id=0)
