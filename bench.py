@@ -103,6 +103,26 @@ def bench_plain_small():
             "records_identical": bool(same), "photon_steps": steps}
 
 
+def bench_gensteps():
+    """Photons generated on the device from 100 000 gensteps (synthetic track
+    segments <= 10 mm in the long-histories box, 200 photons/mm; generator seed
+    1), tally 256 x 100 ns x 64^2: one persistent launch."""
+    from gpu_optical_photons_amd.gensteps import make_gensteps, run_gpu_gensteps
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = c2_params()
+    c = TallyConfig(256, 100.0, 64)
+    g = make_gensteps(100_000, p, photons_per_mm=200.0, seed=1)
+    n = int(g["n_photons"].sum())
+    tu = dict(block_size=512, steps_per_refill=2, queue_chunk=256)
+    run_gpu_gensteps(p, c, g, tuning=tu)
+    ms = []
+    for _ in range(3):
+        _, _, tm, _ = run_gpu_gensteps(p, c, g, tuning=tu)
+        ms.append(tm.kernel_ms)
+    return {"gensteps": len(g), "photons": n, "kernel_ms": min(ms),
+            "photons_per_s": n / (min(ms) * 1e-3), "upload_ms": tm.h2d_ms}
+
+
 def bench_batched():
     """Batched events: 4096 events (box ~U[10,100] mm per axis, surface model
     uniform over polished / black / specular / Lambertian, n_det ~U[1.4,1.6],
@@ -337,6 +357,7 @@ def main():
         line["batched_events"] = bench_batched()
         line["batched_vs_plain"] = (line["batched_events"]["plain_ms"]
                                     / line["batched_events"]["wavefront_ms"])
+        line["gensteps"] = bench_gensteps()
         line["differentiation"] = bench_differentiation()
         line["calibration"] = bench_calibration()
     if not args.no_cpu:

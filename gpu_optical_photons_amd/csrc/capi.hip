@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -173,6 +174,7 @@ struct Context {
   std::string batch_key;
   // Calibration step (forward + chi-square weights + adjoint) graph and buffers.
   DevBuf step_target, step_inv_nt, step_hist, step_loss;
+  DevBuf gs_data, gs_slots;
   hipGraphExec_t step_exec = nullptr;
   std::string step_key;
 
@@ -1006,6 +1008,125 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     const auto td = std::chrono::steady_clock::now();
     HIPCHK(hipMemcpyAsync(event_tally, lo.evt.counts, count_words * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.d2h_ms = ms_since(td);
+    read_counters(c, counters);
+    tm.total_ms = ms_since(t0);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
+    const optphot_sim_params* p, const optphot_genstep* gensteps, uint32_t n_gensteps,
+    uint64_t first_id, const optphot_tally_config* tc, uint64_t* tally,
+    const optphot_records* records, const optphot_wavefront_tuning* tuning,
+    optphot_step_counters* counters, optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    if (!tc || !tally) throw InvalidArgument("gensteps: tally config / output is NULL");
+    if (n_gensteps > 0 && !gensteps) throw InvalidArgument("gensteps: gensteps is NULL");
+    if (records) validate_records(records);
+    uint64_t total = 0;
+    std::vector<DevGenstep> dev(n_gensteps);
+    std::vector<uint2> slots;
+    optphot_wavefront_tuning tu{256, 2, 256, 0, 0, 0};
+    if (tuning) {
+      if (tuning->block_size) tu.block_size = tuning->block_size;
+      if (tuning->steps_per_refill) tu.steps_per_refill = tuning->steps_per_refill;
+      if (tuning->queue_chunk) tu.queue_chunk = tuning->queue_chunk;
+      if (tuning->blocks_per_cu) tu.blocks_per_cu = tuning->blocks_per_cu;
+    }
+    validate_block(tu.block_size, "gensteps");
+    if (tu.queue_chunk < 64 || tu.queue_chunk % 64 != 0 || tu.steps_per_refill == 0)
+      throw InvalidArgument("gensteps: queue_chunk must be a multiple of 64, steps_per_refill >= 1");
+    auto inside = [&](float x, float y, float z) {
+      return x >= -p->hx && x <= p->hx && y >= -p->hy && y <= p->hy && z >= -p->hz && z <= p->hz;
+    };
+    for (uint32_t k = 0; k < n_gensteps; ++k) {
+      const optphot_genstep& g = gensteps[k];
+      if (!inside(g.x0, g.y0, g.z0) || !inside(g.x1, g.y1, g.z1))
+        throw InvalidArgument("gensteps: genstep " + std::to_string(k) +
+                              " has an end point outside the box");
+      if (!(g.t0 >= 0.0f) || !(g.t1 >= 0.0f) || std::isinf(g.t0) || std::isinf(g.t1))
+        throw InvalidArgument("gensteps: genstep times must be finite and >= 0");
+      dev[k] = DevGenstep{g.x0, g.y0, g.z0, g.t0, g.x1, g.y1, g.z1, g.t1, g.n_photons, 0u, total};
+      for (uint64_t c0 = 0; c0 < g.n_photons; c0 += tu.queue_chunk)
+        slots.push_back(make_uint2(k, static_cast<uint32_t>(c0 / tu.queue_chunk)));
+      total += g.n_photons;
+    }
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+    const DevTally T = to_dev_tally(*tc, *p);
+    check_lds(c, T);
+    const size_t lds = static_cast<size_t>(T.words) * 4;
+    const int max_per_cu = std::max(1, genstep_blocks_per_cu(records != nullptr, tu.block_size, lds));
+    const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
+    WavefrontShape s{};
+    s.block = tu.block_size;
+    s.grid = static_cast<int>(std::max<uint64_t>(
+        1, std::min<uint64_t>((total + tu.block_size - 1) / tu.block_size,
+                              static_cast<uint64_t>(per_cu) * c.cu_count)));
+    s.steps_per_refill = tu.steps_per_refill;
+    s.queue_chunk = tu.queue_chunk;
+    if (total / static_cast<uint64_t>(s.grid) >= 0xffffffffull)
+      throw std::runtime_error("too many photons for one tally launch; split the run");
+
+    unsigned long long* scratch = ensure_scratch(c, s.grid, tm.alloc_ms);
+    const auto ta = std::chrono::steady_clock::now();
+    bool grew = c.gs_data.ensure(std::max<size_t>(1, dev.size()) * sizeof(DevGenstep));
+    grew |= c.gs_slots.ensure(std::max<size_t>(1, slots.size()) * sizeof(uint2));
+    grew |= c.partials.ensure(static_cast<size_t>(s.grid) * T.words * sizeof(uint32_t));
+    grew |= c.tally.ensure((static_cast<size_t>(T.words) + 2) * 8);
+    DevRecords rec{};
+    if (records && total > 0) rec = ensure_records(c, total, tm.alloc_ms);
+    if (grew) tm.alloc_ms += ms_since(ta);
+    const auto th = std::chrono::steady_clock::now();
+    if (!dev.empty())
+      HIPCHK(hipMemcpyAsync(c.gs_data.ptr, dev.data(), dev.size() * sizeof(DevGenstep),
+                            hipMemcpyHostToDevice, c.stream));
+    if (!slots.empty())
+      HIPCHK(hipMemcpyAsync(c.gs_slots.ptr, slots.data(), slots.size() * sizeof(uint2),
+                            hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.h2d_ms = ms_since(th);
+
+    LaunchOut lo{};
+    lo.tally = T;
+    lo.tally_on = true;
+    lo.records = records != nullptr && total > 0;
+    lo.rec = rec;
+    lo.partials = c.partials.as<uint32_t>();
+    lo.sums = scratch + kScratchSums;
+    lo.counters = counters ? scratch + kScratchCounters : nullptr;
+    GenstepArgs ga{c.gs_data.as<DevGenstep>(), c.gs_slots.as<uint2>(), slots.size()};
+    launch_zero_u64(scratch, kScratchSums + 2 * static_cast<size_t>(s.grid), c.stream);
+    launch_zero_u64(c.tally.as<unsigned long long>(), T.words + 2u, c.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev[0], c.stream));
+    if (total > 0) {
+      launch_gensteps(to_dev(*p), key_of(*p), first_id, s, ga, scratch + kScratchQueue, lo,
+                      c.stream);
+      HIPCHK(hipGetLastError());
+      launch_tally_reduce(T, s.grid, lo.partials, lo.sums, c.tally.as<unsigned long long>(),
+                          c.stream);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(c.ev[1], c.stream));
+    tm.kernel_ms = c.elapsed(0, 1);
+    const auto td = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(tally, c.tally.ptr, (static_cast<size_t>(T.words) + 2) * 8,
+                          hipMemcpyDeviceToHost, c.stream));
+    if (lo.records) {
+      void* dst[8] = {records->fate, records->n_boundary, records->n_scatter, records->t,
+                      records->x,    records->y,          records->z,         records->path};
+      const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
+      for (int a = 0; a < 8; ++a)
+        HIPCHK(hipMemcpyAsync(dst[a], c.records[a].ptr, total * sz[a], hipMemcpyDeviceToHost,
+                              c.stream));
+    }
     HIPCHK(hipStreamSynchronize(c.stream));
     tm.d2h_ms = ms_since(td);
     read_counters(c, counters);

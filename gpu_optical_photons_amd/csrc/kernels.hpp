@@ -68,6 +68,15 @@ struct LaunchOut {
   bool records, tally_on, events;
 };
 
+// Genstep launch: photons come from gensteps (device array), handed out as
+// queue slots; slot c is chunk `slots[c].y` (photons slots[c].y * chunk ..)
+// of genstep slots[c].x.
+struct GenstepArgs {
+  const DevGenstep* gensteps;
+  const uint2* slots;
+  uint64_t n_slots;
+};
+
 // Adjoint launch: parameter sets (events) e = 0..n_events-1 with counts[e]
 // photons at ids first_id + offsets[e] ..; per-set weights w[e][k] over the
 // time bins k = 0..time_bins (overflow included); output
@@ -150,6 +159,13 @@ void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials
 
 // Fills n u64 words with zero (a kernel, so it may sit inside captured graphs).
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
+
+// Persistent transport of genstep photons into the block tallies (and
+// records at each photon's global index when out.records).
+void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,
+                     const WavefrontShape& shape, const GenstepArgs& g,
+                     unsigned long long* queue, const LaunchOut& out, hipStream_t stream);
+int genstep_blocks_per_cu(bool records, int block, size_t lds);
 
 // Per-set chi-square loss of the batch tally's arrival-time histograms
 // (rows of `words` u32: 5 fates, then bins1 time bins) against target

@@ -463,6 +463,48 @@ __device__ __forceinline__ void emit(const DevParams& P, RNG& rng, Photon& ph) {
   ph.ns = 0u;
 }
 
+// A genstep on the device: emission segment, its time span, and the global
+// index of its first photon (photons of earlier gensteps come first).
+struct DevGenstep {
+  float x0, y0, z0, t0;
+  float x1, y1, z1, t1;
+  uint32_t n_photons, pad;
+  unsigned long long offset;
+};
+
+// Emission from a genstep (no counterpart in the reference, which lists
+// generation from gensteps as future work, README.md:734-735). Draw order:
+// [u along the segment, only for a non-degenerate segment or time span], cos,
+// phi, emission delay; position x0 + u (x1 - x0) per axis, time
+// t0 + u (t1 - t0) + delay. A degenerate genstep with t0 = 0 draws and computes
+// exactly what the point source at (x0, y0, z0) does (transport.hpp:55-67).
+// Needs 4 words in the stream.
+template <class RNG>
+__device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenstep& G, RNG& rng,
+                                             Photon& ph) {
+  ph.p = V3{G.x0, G.y0, G.z0};
+  float t0 = G.t0;
+  if (G.x0 != G.x1 || G.y0 != G.y1 || G.z0 != G.z1 || G.t0 != G.t1) {
+    const float u = to_unit(rng.template peek<0>());
+    ph.p = V3{G.x0 + u * (G.x1 - G.x0), G.y0 + u * (G.y1 - G.y0), G.z0 + u * (G.z1 - G.z0)};
+    t0 = G.t0 + u * (G.t1 - G.t0);
+    rng.pos += 1u;
+  }
+  const float ua = to_unit(rng.template peek<0>());
+  const float ub = to_unit(rng.template peek<1>());
+  const float ut = to_unit(rng.template peek<2>());
+  rng.pos += 3u;
+  const float cos_t = 2.0f * ua - 1.0f;
+  const float sin_t = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));  // 0 or >= 2^-24
+  float sp, cp;
+  p_sincos_turns(ub, sp, cp);
+  ph.d = V3{sin_t * cp, sin_t * sp, cos_t};
+  ph.t = t0 + (P.tau > 0.0f ? -P.tau * p_log(ut) : 0.0f);
+  ph.path = 0.0f;
+  ph.nb = 0u;
+  ph.ns = 0u;
+}
+
 // One transport step (transport.hpp:71-145). Returns true while the photon
 // is alive; otherwise *fate is set. The stream must hold the step's (up to 5)
 // words: Stream after refill() (pos <= 3), RingStream with level() >= 5.

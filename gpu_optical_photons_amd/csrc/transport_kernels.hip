@@ -701,6 +701,113 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Photons from gensteps. The queue and refill are those of batch_kernel
+// (slots claimed kSlotsPerClaim per atomic, each slot one chunk of one
+// genstep); the geometry and tally are the run's (LDS histograms, as
+// wavefront_kernel), and each lane carries its genstep's segment.
+template <bool kRec, bool kCount, int kSurf>
+__global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey key,
+                                                       uint64_t first_id,
+                                                       uint32_t steps_per_refill, uint32_t chunk,
+                                                       GenstepArgs g, unsigned long long* queue,
+                                                       LaunchOut out) {
+  const DevParams& P = P0;
+  extern __shared__ uint32_t lds[];
+  zero_lds(lds, out.tally.words);
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  WaveCounters<kCount> wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;
+
+  uint32_t wg = 0;                 // genstep of the wave's open range (uniform)
+  uint64_t q_next = 0, q_end = 0;  // open range, indices within genstep wg
+  uint64_t s_next = 0, s_end = 0;  // slots claimed by the wave, not yet opened
+  bool q_done = false;
+  bool alive = false;
+  uint64_t idx = 0;  // global photon index
+  uint32_t done = 0, fate = kMaxSteps;
+  Stream rng;
+  Photon ph;
+
+  for (;;) {
+    const uint64_t dead = __ballot(!alive);
+    if (dead != 0 && !q_done) {
+      const uint32_t need = popc(dead);
+      const uint64_t have = q_end - q_next;
+      const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+      const uint64_t b1 = q_next;
+      const uint32_t g1 = wg;
+      q_next += t1;
+      uint64_t b2 = 0;
+      uint32_t t2 = 0;
+      if (t1 < need) {
+        if (s_next >= s_end) {
+          unsigned long long c = 0;
+          if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(kSlotsPerClaim));
+          c = first_lane_u64(c);
+          if (c >= g.n_slots) {
+            q_done = true;
+          } else {
+            s_next = c;
+            s_end = (c + kSlotsPerClaim < g.n_slots) ? c + kSlotsPerClaim : g.n_slots;
+          }
+        }
+        if (!q_done) {
+          const uint2 sl = g.slots[s_next++];
+          wg = sl.x;
+          const uint64_t ng = g.gensteps[wg].n_photons;
+          q_next = static_cast<uint64_t>(sl.y) * chunk;
+          q_end = (q_next + chunk < ng) ? q_next + chunk : ng;
+          t2 = (need - t1 < q_end - q_next) ? need - t1 : static_cast<uint32_t>(q_end - q_next);
+          b2 = q_next;
+          q_next += t2;
+        }
+      }
+      const uint32_t r = rank_in(dead);
+      bool fresh = false;
+      if (!alive && r < t1 + t2) {
+        const DevGenstep G = g.gensteps[r < t1 ? g1 : wg];
+        idx = G.offset + (r < t1 ? b1 + r : b2 + (r - t1));
+        rng.start(first_id + idx, key, 0u);
+        emit_genstep(P, G, rng, ph);
+        done = 0;
+        fate = kMaxSteps;
+        alive = P.max_steps > 0u;
+        fresh = true;
+      }
+      wc.add(kCntEmitted, t1 + t2);
+      if (fresh && !alive) finish<kRec, true>(P, out, lds, idx, ph, fate, snb, sns);
+    }
+    if (__ballot(alive) == 0) {
+      if (q_done) break;
+      continue;
+    }
+    for (uint32_t k = 0; k < steps_per_refill; ++k) {
+      const uint64_t act = __ballot(alive);
+      if (act == 0) break;
+      StepInfo info{};
+      bool finished = false;
+      if (alive) {
+        rng.refill(key);
+        const bool still = step<kSurf>(P, rng, ph, fate, info);
+        ++done;
+        if (!still || done >= P.max_steps) {
+          if (still) fate = kMaxSteps;
+          finished = true;
+          alive = false;
+        }
+      }
+      wc.add_step(act, info);
+      if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+      if (finished) finish<kRec, true>(P, out, lds, idx, ph, fate, snb, sns);
+    }
+  }
+  __syncthreads();
+  flush_block<true, kCount>(out, lds, snb, sns, wc);
+}
+
 // Column sums of the [blocks][words] partial histograms (and the [blocks][2]
 // sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
 // every gridDim.y-th block row, so enough waves are in flight to stream the
@@ -1305,6 +1412,27 @@ void launch_chi2_weights(const uint32_t* counts, uint32_t words, uint32_t bins1,
                      bins1, offsets, target, inv_nt, hist, loss, weights);
 }
 
+template <bool R, bool C, int S>
+void launch_gensteps_t(const DevParams& P, PhiloxKey key, uint64_t first_id,
+                       const WavefrontShape& s, const GenstepArgs& g, unsigned long long* queue,
+                       const LaunchOut& out, hipStream_t stream) {
+  hipLaunchKernelGGL((genstep_kernel<R, C, S>), dim3(s.grid), dim3(s.block),
+                     static_cast<size_t>(out.tally.words) * sizeof(uint32_t), stream, P, key,
+                     first_id, s.steps_per_refill, s.queue_chunk, g, queue, out);
+}
+
+template <bool R, bool C>
+void launch_gensteps_surf(const DevParams& P, PhiloxKey key, uint64_t first_id,
+                          const WavefrontShape& s, const GenstepArgs& g,
+                          unsigned long long* queue, const LaunchOut& out, hipStream_t stream) {
+  switch (P.surface) {
+    case kPolished: launch_gensteps_t<R, C, kPolished>(P, key, first_id, s, g, queue, out, stream); break;
+    case kBlack: launch_gensteps_t<R, C, kBlack>(P, key, first_id, s, g, queue, out, stream); break;
+    case kSpecular: launch_gensteps_t<R, C, kSpecular>(P, key, first_id, s, g, queue, out, stream); break;
+    default: launch_gensteps_t<R, C, kLambertian>(P, key, first_id, s, g, queue, out, stream); break;
+  }
+}
+
 void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
                        hipStream_t stream) {
   hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, stream, out, iters,
@@ -1372,6 +1500,25 @@ size_t batch_lds_bytes(uint32_t n_events, uint32_t words) {
 size_t batch_ring_lds_bytes(size_t tally_lds, int block) {
   return ((tally_lds + 15u) & ~size_t{15}) +
          static_cast<size_t>(RingStream::kSlots) * block * sizeof(uint32_t);
+}
+
+void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,
+                     const WavefrontShape& shape, const GenstepArgs& g,
+                     unsigned long long* queue, const LaunchOut& out, hipStream_t stream) {
+  const bool count = out.counters != nullptr;
+  if (out.records && count)
+    launch_gensteps_surf<true, true>(P, key, first_id, shape, g, queue, out, stream);
+  else if (out.records)
+    launch_gensteps_surf<true, false>(P, key, first_id, shape, g, queue, out, stream);
+  else if (count)
+    launch_gensteps_surf<false, true>(P, key, first_id, shape, g, queue, out, stream);
+  else
+    launch_gensteps_surf<false, false>(P, key, first_id, shape, g, queue, out, stream);
+}
+
+int genstep_blocks_per_cu(bool records, int block, size_t lds) {
+  return records ? blocks_per_cu(genstep_kernel<true, false, kLambertian>, block, lds)
+                 : blocks_per_cu(genstep_kernel<false, false, kLambertian>, block, lds);
 }
 
 int batch_blocks_per_cu(int block, size_t lds, bool ring) {

@@ -228,6 +228,34 @@ int optphot_run_gpu_adjoint(const optphot_sim_params* params, const uint64_t* ph
                             float time_max, const double* weights, double* grad,
                             const optphot_wavefront_tuning* tuning, optphot_run_timing* timing);
 
+/* A scintillation step summary ("genstep"): photons are emitted uniformly
+ * along the segment (x0,y0,z0) at t0 -> (x1,y1,z1) at t1, isotropically,
+ * each with the run's Exp(tau) emission delay added. Both end points must lie
+ * in the box. 40 bytes. */
+typedef struct optphot_genstep {
+  float x0, y0, z0, t0;
+  float x1, y1, z1, t1;
+  uint32_t n_photons;
+  uint32_t reserved; /* 0 */
+} optphot_genstep;
+
+/* Photons generated on the device from gensteps and transported in the box
+ * of `p` (its source fields are unused). Photon k of genstep g has global
+ * index i = n_0 + ... + n_{g-1} + k and Philox id first_id + i. Draws per
+ * photon: [u along the segment, only if the segment or time span is
+ * non-degenerate], cos, phi, emission delay, then the transport steps. A
+ * degenerate genstep with t0 = 0 is the point source at (x0, y0, z0), photon
+ * for photon. Writes the tally (as optphot_run_gpu_tally) and, when `records`
+ * is non-NULL, records of all photons at their global index (host arrays of
+ * the total photon count). No counterpart in the reference, which lists
+ * generation from gensteps as future work (README.md:734-735). */
+int optphot_run_gpu_gensteps(const optphot_sim_params* p, const optphot_genstep* gensteps,
+                             uint32_t n_gensteps, uint64_t first_id,
+                             const optphot_tally_config* c, uint64_t* tally,
+                             const optphot_records* records,
+                             const optphot_wavefront_tuning* tuning,
+                             optphot_step_counters* counters, optphot_run_timing* timing);
+
 /* One calibration step for a batch of parameter sets, as a single captured
  * hipGraph replayed while the shapes are unchanged: the batched forward
  * launch (as optphot_run_gpu_batch), the per-set chi-square loss of the

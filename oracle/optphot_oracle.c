@@ -29,6 +29,7 @@
  * build.
  */
 #include <math.h>
+#include <stdlib.h>
 #include <stdint.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -569,6 +570,53 @@ static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int
   return 1;
 }
 
+/* Emission from a genstep (no counterpart in the reference, whose README.md
+   lists generation from gensteps as future work). Draw order: [u along the
+   segment, only if the segment or its time span is non-degenerate], cos, phi,
+   emission delay. A degenerate genstep with t0 = 0 therefore draws and
+   computes exactly what the reference's point source at (x0, y0, z0) does
+   (transport.hpp:55-67): same words, and t = 0 + delay = delay. */
+static pstate start_genstep_photon(const oracle_params* p, const oracle_genstep* g,
+                                   stream_t* rng) {
+  pstate s;
+  s.pos = v3_make(g->x0, g->y0, g->z0);
+  float t0 = g->t0;
+  if (g->x0 != g->x1 || g->y0 != g->y1 || g->z0 != g->z1 || g->t0 != g->t1) {
+    const float u = stream_uniform(rng);
+    s.pos = v3_make(g->x0 + u * (g->x1 - g->x0), g->y0 + u * (g->y1 - g->y0),
+                    g->z0 + u * (g->z1 - g->z0));
+    t0 = g->t0 + u * (g->t1 - g->t0);
+  }
+  const float ua = stream_uniform(rng);
+  const float ub = stream_uniform(rng);
+  s.dir = sample_isotropic(ua, ub);
+  s.t = t0 + sample_emission_time(p->tau, stream_uniform(rng));
+  s.path = 0.0f;
+  s.nb = 0u;
+  s.ns = 0u;
+  return s;
+}
+
+static void transport_genstep_photon(const oracle_params* p, const oracle_genstep* g,
+                                     uint64_t id, oracle_result* r) {
+  stream_t rng;
+  stream_init(&rng, p->seed, id, 0u);
+  pstate s = start_genstep_photon(p, g, &rng);
+  int fate = ORACLE_FATE_MAX_STEPS;
+  for (uint32_t step = 0; step < p->max_steps; ++step) {
+    if (!step_photon(p, &rng, &s, &fate)) break;
+  }
+  r->fate = (uint8_t)fate;
+  r->n_boundary = s.nb;
+  r->n_scatter = s.ns;
+  r->t = s.t;
+  r->x = s.pos.x;
+  r->y = s.pos.y;
+  r->z = s.pos.z;
+  r->path = s.path;
+  r->draws = stream_draws(&rng);
+}
+
 /* transport.hpp:151-159 */
 static void transport_photon(const oracle_params* p, uint64_t id, oracle_result* r) {
   stream_t rng;
@@ -730,6 +778,68 @@ ORACLE_EXPORT uint64_t oracle_tally_words(const oracle_tally_config* c) {
 
 /* cpu_backend.cpp:28-36 + 115-147: every fate counted, histograms for
    detected photons only, integer sums over all photons. */
+/* Photons of gensteps g = 0 .. n_g-1: photon k of genstep g has index
+   i = n_0 + ... + n_{g-1} + k and Philox id first_id + i. Records (arrays may
+   be NULL) are written at i; the tally (NULL to skip) counts all photons'
+   fates and sums and the detected photons' histograms. */
+ORACLE_EXPORT void oracle_run_gensteps(const oracle_params* p, const oracle_genstep* g,
+                                       uint32_t n_g, uint64_t first_id,
+                                       const oracle_tally_config* c, uint64_t* tally,
+                                       uint8_t* fate, uint32_t* n_boundary, uint32_t* n_scatter,
+                                       float* t, float* x, float* y, float* z, float* path,
+                                       int threads) {
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < n_g; ++k) total += g[k].n_photons;
+  uint64_t* off = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)n_g + 1));
+  off[0] = 0;
+  for (uint32_t k = 0; k < n_g; ++k) off[k + 1] = off[k] + g[k].n_photons;
+  uint64_t words = 0, time_off = 0, xy_off = 0, sums_off = 0;
+  if (tally) {
+    words = oracle_tally_words(c);
+    time_off = 5u;
+    xy_off = 5u + c->time_bins + 1u;
+    sums_off = xy_off + (uint64_t)c->xy_bins * c->xy_bins;
+    for (uint64_t i = 0; i < words; ++i) tally[i] = 0;
+  }
+  const int nt = thread_count(threads);
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(nt)
+  for (int64_t i = 0; i < (int64_t)total; ++i) {
+    uint32_t lo = 0, hi = n_g; /* genstep: off[lo] <= i < off[lo + 1] */
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (off[mid] <= (uint64_t)i) lo = mid; else hi = mid;
+    }
+    while (off[lo + 1] <= (uint64_t)i) ++lo; /* skip empty gensteps */
+    oracle_result r;
+    transport_genstep_photon(p, &g[lo], first_id + (uint64_t)i, &r);
+    if (fate) {
+      fate[i] = r.fate;
+      n_boundary[i] = r.n_boundary;
+      n_scatter[i] = r.n_scatter;
+      t[i] = r.t;
+      x[i] = r.x;
+      y[i] = r.y;
+      z[i] = r.z;
+      path[i] = r.path;
+    }
+    if (tally) {
+#pragma omp atomic
+      tally[r.fate] += 1;
+#pragma omp atomic
+      tally[sums_off] += r.n_boundary;
+#pragma omp atomic
+      tally[sums_off + 1] += r.n_scatter;
+      if (r.fate == ORACLE_FATE_DETECTED) {
+#pragma omp atomic
+        tally[time_off + time_bin(r.t, c)] += 1;
+#pragma omp atomic
+        tally[xy_off + xy_bin(r.x, r.y, p, c)] += 1;
+      }
+    }
+  }
+  free(off);
+}
+
 ORACLE_EXPORT void oracle_run_tally(const oracle_params* p, const oracle_tally_config* c,
                                     uint64_t n, uint64_t first_id, int threads, uint64_t* out) {
   const uint64_t words = oracle_tally_words(c);

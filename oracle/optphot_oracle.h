@@ -33,6 +33,16 @@ typedef struct {
   uint32_t xy_bins;
 } oracle_tally_config;
 
+/* A scintillation step summary: photons are emitted uniformly along the
+ * segment (x0,y0,z0,t0) -> (x1,y1,z1,t1), isotropically, with an Exp(tau)
+ * emission delay. 40 bytes. */
+typedef struct {
+  float x0, y0, z0, t0;
+  float x1, y1, z1, t1;
+  uint32_t n_photons;
+  uint32_t reserved;
+} oracle_genstep;
+
 typedef struct {
   uint8_t fate;
   uint32_t n_boundary, n_scatter;

@@ -133,6 +133,26 @@ class Oracle(_Lib):
                                    ctypes.c_void_p, ctypes.c_int])
         self._fn("trace", ctypes.c_uint32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
                                             _u8p, _f32p, ctypes.c_void_p])
+        self._fn("run_gensteps", None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                        ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int])
+
+    def gensteps(self, params, gensteps, first_id=0, time_bins=100, time_max=50.0, xy_bins=50,
+                 records=False, threads=0):
+        """Tally (dict) and, with records=True, records of photons from gensteps
+        (structured array with fields x0 y0 z0 t0 x1 y1 z1 t1 n_photons reserved)."""
+        g = np.ascontiguousarray(gensteps)
+        assert g.dtype.itemsize == 40
+        total = int(g["n_photons"].sum())
+        cfg = TallyConfigC(time_bins, time_max, xy_bins)
+        flat = np.zeros(tally_words(time_bins, xy_bins), np.uint64)
+        r = _empty_records(total) if records else None
+        ptrs = [r[k].ctypes.data if records else None for k in RECORD_FIELDS]
+        self.run_gensteps(ctypes.byref(params), g.ctypes.data, len(g), first_id,
+                          ctypes.byref(cfg), flat.ctypes.data, *ptrs, threads)
+        return split_tally(flat, time_bins, xy_bins), r
 
     def step_trace(self, params, photon_id, cap=1 << 14, with_state=False):
         """(codes u8 [steps], margins f32 [steps]) of one photon: per step the

@@ -1,0 +1,121 @@
+"""Photons from gensteps (optphot_run_gpu_gensteps; the reference lists
+generation from gensteps as future work, README.md:734-735, so there is no
+reference kernel to restate).
+
+How the emission is pinned:
+* a degenerate genstep (x0 = x1, t0 = t1 = 0) draws and computes what the
+  reference's point source does: its photons equal the reference's own CPU
+  path (oracle/_ref, portable math) record for record;
+* along a segment, with absorption so short that photons stop where they are
+  emitted, the emission point's fraction along the segment is U[0, 1]
+  (Kolmogorov-Smirnov) and its time is t0 + u (t1 - t0) plus an Exp(tau)
+  delay (mean and KS);
+* global photon indices: re-running a tail of the genstep list with
+  first_id = its offset reproduces the tail of the full run;
+* GPU: the kernel equals the oracle bit for bit (tally and records), for any
+  launch shape.
+"""
+import numpy as np
+import pytest
+from scipy import stats
+
+from gpu_optical_photons_amd.gensteps import GENSTEP_DTYPE, make_gensteps
+from gpu_optical_photons_amd.params import SimParams
+from tests.configs import PARITY
+
+FIELDS = ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path")
+
+
+def _point(p, n):
+    g = np.zeros(1, GENSTEP_DTYPE)
+    g["x0"] = g["x1"] = p.src_x
+    g["y0"] = g["y1"] = p.src_y
+    g["z0"] = g["z1"] = p.src_z
+    g["n_photons"] = n
+    return g
+
+
+@pytest.mark.parametrize("name", ["c1_plain", "every_process", "specular"])
+def test_degenerate_genstep_is_the_reference_point_source(oracle_portable, name):
+    from oracle.oracle import Oracle, Reference, reference_available
+    p = PARITY[name].copy(source=0)
+    ref_lib = Reference("portable") if reference_available("portable") else oracle_portable
+    n = 20_000
+    _, r = oracle_portable.gensteps(p, _point(p, n), first_id=77, records=True)
+    ref = ref_lib.records(p, n, first_id=77)
+    for f in FIELDS:
+        assert np.array_equal(r[f].view(np.uint8), ref[f].view(np.uint8)), f
+
+
+def test_emission_along_segment(oracle_portable):
+    p = SimParams(hx=50.0, hy=50.0, hz=50.0, abs_length=1e-6, tau=3.0, seed=3)
+    g = np.zeros(1, GENSTEP_DTYPE)
+    g["x0"], g["y0"], g["z0"], g["t0"] = -20.0, 5.0, -10.0, 1.0
+    g["x1"], g["y1"], g["z1"], g["t1"] = 30.0, -15.0, 25.0, 1.5
+    n = 100_000
+    g["n_photons"] = n
+    _, r = oracle_portable.gensteps(p, g, records=True)
+    assert np.all(r["fate"] == 1)  # absorbed in the bulk, within ~1e-6 mm of emission
+    a = np.array([-20.0, 5.0, -10.0])
+    b = np.array([30.0, -15.0, 25.0])
+    pos = np.stack([r["x"], r["y"], r["z"]], axis=1).astype(np.float64)
+    u = (pos - a) @ (b - a) / ((b - a) @ (b - a))
+    assert np.abs(np.linalg.norm((pos - a) - np.outer(u, b - a), axis=1)).max() < 1e-3
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    delay = r["t"].astype(np.float64) - (1.0 + u * 0.5) - r["path"] * 1.63 / 299.792458
+    assert abs(delay.mean() - 3.0) < 5 * 3.0 / np.sqrt(n)
+    assert stats.kstest(delay, "expon", args=(0.0, 3.0)).pvalue > 1e-3
+
+
+def test_global_index_and_first_id(oracle_portable):
+    p = PARITY["c2_long"]
+    g = make_gensteps(60, p, photons_per_mm=50.0, seed=4)
+    g["n_photons"][3] = 0  # an empty genstep
+    _, full = oracle_portable.gensteps(p, g, first_id=1000, records=True)
+    off = int(g["n_photons"][:20].sum())
+    _, tail = oracle_portable.gensteps(p, g[20:], first_id=1000 + off, records=True)
+    for f in FIELDS:
+        assert np.array_equal(tail[f], full[f][off:]), f
+
+
+def test_gensteps_outside_the_box_rejected_before_gpu():
+    from gpu_optical_photons_amd import backend
+    from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = SimParams(hx=10.0, hy=10.0, hz=10.0)
+    g = np.zeros(1, GENSTEP_DTYPE)
+    g["x1"] = 11.0
+    g["n_photons"] = 10
+    with pytest.raises(ValueError, match="outside the box"):
+        run_gpu_gensteps(p, TallyConfig(), g)
+    g["x1"] = 0.0
+    g["t0"] = -1.0
+    with pytest.raises(ValueError, match="times"):
+        run_gpu_gensteps(p, TallyConfig(), g)
+    assert backend is not None
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["c2_long", "every_process", "c1_plain"])
+def test_gpu_gensteps_equal_oracle(backend, oracle_portable, name):
+    from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = PARITY[name]
+    g = make_gensteps(400, p, photons_per_mm=200.0, seed=9)
+    g["n_photons"][7] = 0
+    g[11]["x1"], g[11]["y1"], g[11]["z1"], g[11]["t1"] = g[11]["x0"], g[11]["y0"], g[11]["z0"], g[11]["t0"]
+    c = TallyConfig(256, 100.0, 64)
+    print(f"[gensteps] {name}: {len(g)} gensteps, {int(g['n_photons'].sum())} photons", flush=True)
+    ref_t, ref_r = oracle_portable.gensteps(p, g, first_id=5, time_bins=256, time_max=100.0,
+                                            xy_bins=64, records=True)
+    for tu in (None, dict(block_size=64, steps_per_refill=1, queue_chunk=64),
+               dict(block_size=1024, steps_per_refill=8, queue_chunk=1024)):
+        t, r, tm, cnt = run_gpu_gensteps(p, c, g, first_id=5, records=True, tuning=tu,
+                                         counters=True)
+        assert np.array_equal(t.fate_counts, ref_t["fate_counts"])
+        assert np.array_equal(t.time_hist, ref_t["time_hist"])
+        assert np.array_equal(t.xy_hist, ref_t["xy_hist"])
+        assert t.boundary_sum == ref_t["boundary_sum"] and t.scatter_sum == ref_t["scatter_sum"]
+        for f in FIELDS:
+            assert np.array_equal(getattr(r, f).view(np.uint8), ref_r[f].view(np.uint8)), f
+        assert cnt.as_dict()["emitted"] == int(g["n_photons"].sum())
