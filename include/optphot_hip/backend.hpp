@@ -217,6 +217,31 @@ inline RunTiming run_gpu_records_device(const SimParams& p, uint64_t n,
   return tm;
 }
 
+// Photons generated on the device from gensteps (see optphot_run_gpu_gensteps):
+// the run's tally, and with `records` the records of all photons at their
+// global index.
+using Genstep = optphot_genstep;
+inline RunTiming run_gpu_gensteps(const SimParams& p, const TallyConfig& c,
+                                  const std::vector<Genstep>& gensteps, Tally& out,
+                                  uint64_t first_id = 0, PhotonRecords* records = nullptr,
+                                  const WavefrontTuning* tuning = nullptr) {
+  std::vector<uint64_t> w(detail::tally_words(c));
+  uint64_t total = 0;
+  for (const Genstep& g : gensteps) total += g.n_photons;
+  optphot_records v{};
+  if (records) {
+    records->resize(total);
+    v = detail::view(*records);
+  }
+  RunTiming tm;
+  detail::check(optphot_run_gpu_gensteps(
+      c_params(p), gensteps.data(), static_cast<uint32_t>(gensteps.size()), first_id,
+      c_tally_config(c), w.data(), records ? &v : nullptr, tuning, nullptr,
+      reinterpret_cast<optphot_run_timing*>(&tm)));
+  detail::unpack(w, c, out);
+  return tm;
+}
+
 // Per-event results of a batch: 5 fate counts, then the arrival-time
 // histogram of detected photons (time_bins + 1 bins, last = overflow).
 struct EventTally {

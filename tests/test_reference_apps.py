@@ -84,3 +84,21 @@ def test_reference_main_app_gpu_equals_cpu(tmp_path, mode):
         b = np.load(tmp_path / "gpu" / f"{f}.npy")
         assert a.dtype == b.dtype and a.shape == b.shape, f
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f
+
+
+def test_compare_tool_on_reference_cpu_and_library_gpu_runs(tmp_path):
+    """The reference's optphot with its CPU backend and this library's optphot
+    on the GPU write records of the same photons; this library's
+    optphot_compare reads both directories and finds them bitwise identical."""
+    import json
+    ref_app = _app("optphot")
+    own = os.path.join(ROOT, "gpu_optical_photons_amd", "bin")
+    common = [*C2, "--n_photons", "200000", "--mode", "records"]
+    _run([ref_app, *common, "--backend", "cpu", "--threads", "0", "--out",
+          str(tmp_path / "cpu")], tmp_path)
+    _run([os.path.join(own, "optphot"), *common, "--kernel", "wavefront", "--out",
+          str(tmp_path / "gpu")], tmp_path)
+    _run([os.path.join(own, "optphot_compare"), str(tmp_path / "cpu"), str(tmp_path / "gpu"),
+          "--out", str(tmp_path / "cmp.json")], tmp_path)
+    h = json.loads((tmp_path / "cmp.json").read_text())["history"]
+    assert h["bitwise_identical"] == h["n"] == 200000
