@@ -923,7 +923,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     if (ring && batch_ring_lds_bytes(batch_lds_bytes(n_events, words), tu.block_size) > c.max_lds)
       throw InvalidArgument("batch: event tallies plus LDS streams exceed the LDS of a block; use "
                             "a smaller block_size");
-    const WavefrontShape s = [&] {
+    WavefrontShape s = [&] {
       const int max_per_cu = std::max(
           1, batch_blocks_per_cu(tu.block_size, batch_lds_bytes(n_events, words), ring));
       const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
@@ -966,13 +966,15 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     for (uint32_t e = 0; e < n_events; ++e)
       max_chunks = std::max<uint64_t>(max_chunks, (photons[e] + s.queue_chunk - 1) / s.queue_chunk);
     const uint64_t n_slots = max_chunks * n_events;
+    s.slots_per_claim = slots_per_claim(n_slots, static_cast<uint64_t>(s.grid) * s.block / 64u);
     if (method == OPTPHOT_BATCH_WAVEFRONT) {
       const std::string gkey = std::to_string(n_events) + "/" + std::to_string(n_slots) + "/" +
                                std::to_string(first_id) + "/" + std::to_string(params[0].seed) +
                                "/" + std::to_string(words) + "/" + std::to_string(lo.evt.time_scale) +
                                "/" + std::to_string(s.block) + "/" + std::to_string(s.grid) + "/" +
                                std::to_string(s.steps_per_refill) + "/" +
-                               std::to_string(s.queue_chunk) + (counters ? "/c" : "") +
+                               std::to_string(s.queue_chunk) + "/" +
+                               std::to_string(s.slots_per_claim) + (counters ? "/c" : "") +
                                (s.stream_in_lds ? "/ring" : "");
       if (!c.batch_exec || c.batch_key != gkey) {
         if (c.batch_exec) HIPCHK(hipGraphExecDestroy(c.batch_exec));
@@ -1071,6 +1073,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
                               static_cast<uint64_t>(per_cu) * c.cu_count)));
     s.steps_per_refill = tu.steps_per_refill;
     s.queue_chunk = tu.queue_chunk;
+    s.slots_per_claim = slots_per_claim(slots.size(), static_cast<uint64_t>(s.grid) * s.block / 64u);
     if (total / static_cast<uint64_t>(s.grid) >= 0xffffffffull)
       throw std::runtime_error("too many photons for one tally launch; split the run");
 
@@ -1250,6 +1253,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
       max_chunks =
           std::max<uint64_t>(max_chunks, (photons[e] + sf.queue_chunk - 1) / sf.queue_chunk);
     const uint64_t n_slots = max_chunks * n_sets;
+    sf.slots_per_claim = slots_per_claim(n_slots, static_cast<uint64_t>(sf.grid) * sf.block / 64u);
 
     // adjoint shape and arguments
     WavefrontShape sb{};
@@ -1286,8 +1290,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
         "/" + std::to_string(params[0].seed) + "/" + std::to_string(words) + "/" +
         std::to_string(lo.evt.time_scale) + "/" + std::to_string(surface) + "/f" +
         std::to_string(sf.block) + "," + std::to_string(sf.grid) + "," +
-        std::to_string(sf.steps_per_refill) + "," + std::to_string(sf.queue_chunk) +
-        (sf.stream_in_lds ? ",ring" : "") + "/b" +
+        std::to_string(sf.steps_per_refill) + "," + std::to_string(sf.queue_chunk) + "," +
+        std::to_string(sf.slots_per_claim) + (sf.stream_in_lds ? ",ring" : "") + "/b" +
         std::to_string(sb.block) + "," + std::to_string(sb.grid) + "," +
         std::to_string(sb.steps_per_refill) + "," + std::to_string(sb.queue_chunk);
     if (!c.step_exec || c.step_key != gkey) {

@@ -104,7 +104,17 @@ struct WavefrontShape {
   // Tally launches: keep each lane's Philox window in an LDS ring (takes
   // precedence over successor_threshold).
   uint32_t stream_in_lds = 0;
+  // Batch / genstep launches: queue slots a wave claims per atomic.
+  uint32_t slots_per_claim = 1;
 };
+
+// Slots per claim for a slot queue of n_slots drained by `waves` waves: up to
+// 16 (fewer atomics on the one queue counter), but small enough that every
+// wave gets at least 4 claims, so a small batch still reaches every wave.
+inline uint32_t slots_per_claim(uint64_t n_slots, uint64_t waves) {
+  const uint64_t k = n_slots / (4u * (waves ? waves : 1u));
+  return static_cast<uint32_t>(k < 1u ? 1u : (k > 16u ? 16u : k));
+}
 
 // History-based launches: one lane per photon (records), or a grid-stride
 // loop over photons (tally). first_id offsets the Philox photon ids; records

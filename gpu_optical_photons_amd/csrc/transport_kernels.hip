@@ -57,11 +57,12 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 
 __device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
 
-// Batched events: queue slots a wave claims per atomic. Slot c is chunk c / E
-// of event c mod E, so a wave's range spans kSlotsPerClaim different events;
-// many slots are empty (events with fewer chunks than the largest), and those
-// cost the wave a compare instead of a contended global atomic.
-constexpr uint32_t kSlotsPerClaim = 16;
+// Batched events and gensteps: a wave claims `per_claim` consecutive queue
+// slots per atomic (slots_per_claim() on the host: at most 16, and few enough
+// that every wave gets at least 4 claims). For events, slot c is chunk c / E of
+// event c mod E, so a claim spans several events; empty slots (events with
+// fewer chunks than the largest) cost the wave a compare instead of a
+// contended global atomic.
 
 // Wave-uniform branch-event counters (kept in SGPRs). Compiled in only for
 // the launches that report them: 16 live 64-bit scalars otherwise push the
@@ -571,6 +572,7 @@ template <bool kCount, bool kLds>
 __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_t first_id,
                                                           uint32_t steps_per_refill,
                                                           uint32_t chunk, uint64_t n_slots,
+                                                          uint32_t per_claim,
                                                           unsigned long long* queue,
                                                           LaunchOut out) {
   extern __shared__ uint32_t lds[];
@@ -624,14 +626,14 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
         while (t1 < need && !q_done) {  // uniform: take the next non-empty slot
           if (s_next >= s_end) {  // the wave's slot range is used up: claim the next one
             unsigned long long c = 0;
-            if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(kSlotsPerClaim));
+            if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(per_claim));
             c = first_lane_u64(c);
             if (c >= n_slots) {
               q_done = true;
               break;
             }
             s_next = c;
-            s_end = (c + kSlotsPerClaim < n_slots) ? c + kSlotsPerClaim : n_slots;
+            s_end = (c + per_claim < n_slots) ? c + per_claim : n_slots;
           }
           const uint64_t c = s_next++;
           const uint32_t e = static_cast<uint32_t>(c % E);
@@ -703,14 +705,15 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
 
 // ---------------------------------------------------------------------------
 // Photons from gensteps. The queue and refill are those of batch_kernel
-// (slots claimed kSlotsPerClaim per atomic, each slot one chunk of one
+// (slots claimed `per_claim` per atomic, each slot one chunk of one
 // genstep); the geometry and tally are the run's (LDS histograms, as
 // wavefront_kernel), and each lane carries its genstep's segment.
 template <bool kRec, bool kCount, int kSurf>
 __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey key,
                                                        uint64_t first_id,
                                                        uint32_t steps_per_refill, uint32_t chunk,
-                                                       GenstepArgs g, unsigned long long* queue,
+                                                       uint32_t per_claim, GenstepArgs g,
+                                                       unsigned long long* queue,
                                                        LaunchOut out) {
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
@@ -745,13 +748,13 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
       if (t1 < need) {
         if (s_next >= s_end) {
           unsigned long long c = 0;
-          if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(kSlotsPerClaim));
+          if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(per_claim));
           c = first_lane_u64(c);
           if (c >= g.n_slots) {
             q_done = true;
           } else {
             s_next = c;
-            s_end = (c + kSlotsPerClaim < g.n_slots) ? c + kSlotsPerClaim : g.n_slots;
+            s_end = (c + per_claim < g.n_slots) ? c + per_claim : g.n_slots;
           }
         }
         if (!q_done) {
@@ -826,8 +829,8 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
 template <bool kCount, bool kLds>
 __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t first_id,
                                                      uint32_t steps_per_refill, uint32_t chunk,
-                                                     uint64_t n_slots, unsigned long long* queue,
-                                                     LaunchOut out) {
+                                                     uint64_t n_slots, uint32_t per_claim,
+                                                     unsigned long long* queue, LaunchOut out) {
   extern __shared__ uint32_t lds[];
   const uint32_t lds_words = kLds ? out.evt.n_events * out.evt.words : 0u;
   if (kLds) {
@@ -869,14 +872,14 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
       while (t1 < need && !q_done) {  // uniform: take the next non-empty slot
         if (s_next >= s_end) {  // the wave's slot range is used up: claim the next one
           unsigned long long c = 0;
-          if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(kSlotsPerClaim));
+          if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(per_claim));
           c = first_lane_u64(c);
           if (c >= n_slots) {
             q_done = true;
             break;
           }
           s_next = c;
-          s_end = (c + kSlotsPerClaim < n_slots) ? c + kSlotsPerClaim : n_slots;
+          s_end = (c + per_claim < n_slots) ? c + per_claim : n_slots;
         }
         const uint64_t c = s_next++;
         const uint32_t e = static_cast<uint32_t>(c % E);
@@ -1357,34 +1360,38 @@ void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
     const size_t bytes = batch_ring_lds_bytes(lds, s.block);
     if (lds && out.counters)
       hipLaunchKernelGGL((batch_ring_kernel<true, true>), dim3(s.grid), dim3(s.block), bytes,
-                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
-                         out);
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
+                         s.slots_per_claim, queue, out);
     else if (lds)
       hipLaunchKernelGGL((batch_ring_kernel<false, true>), dim3(s.grid), dim3(s.block), bytes,
-                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
-                         out);
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
+                         s.slots_per_claim, queue, out);
     else if (out.counters)
       hipLaunchKernelGGL((batch_ring_kernel<true, false>), dim3(s.grid), dim3(s.block), bytes,
-                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
-                         out);
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
+                         s.slots_per_claim, queue, out);
     else
       hipLaunchKernelGGL((batch_ring_kernel<false, false>), dim3(s.grid), dim3(s.block), bytes,
-                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue,
-                         out);
+                         stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
+                         s.slots_per_claim, queue, out);
     return;
   }
   if (lds && out.counters)
     hipLaunchKernelGGL((batch_kernel<true, true>), dim3(s.grid), dim3(s.block), lds, stream, key,
-                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
+                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, s.slots_per_claim,
+                       queue, out);
   else if (lds)
     hipLaunchKernelGGL((batch_kernel<false, true>), dim3(s.grid), dim3(s.block), lds, stream,
-                       key, first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
+                       key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
+                       s.slots_per_claim, queue, out);
   else if (out.counters)
     hipLaunchKernelGGL((batch_kernel<true, false>), dim3(s.grid), dim3(s.block), 0, stream, key,
-                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
+                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, s.slots_per_claim,
+                       queue, out);
   else
     hipLaunchKernelGGL((batch_kernel<false, false>), dim3(s.grid), dim3(s.block), 0, stream, key,
-                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, queue, out);
+                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, s.slots_per_claim,
+                       queue, out);
 }
 
 void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
@@ -1418,7 +1425,8 @@ void launch_gensteps_t(const DevParams& P, PhiloxKey key, uint64_t first_id,
                        const LaunchOut& out, hipStream_t stream) {
   hipLaunchKernelGGL((genstep_kernel<R, C, S>), dim3(s.grid), dim3(s.block),
                      static_cast<size_t>(out.tally.words) * sizeof(uint32_t), stream, P, key,
-                     first_id, s.steps_per_refill, s.queue_chunk, g, queue, out);
+                     first_id, s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g, queue,
+                     out);
 }
 
 template <bool R, bool C>
