@@ -63,6 +63,23 @@ class WavefrontTuning(ctypes.Structure):
                 ("successor_threshold", ctypes.c_uint32), ("stream_in_lds", ctypes.c_uint32)]
 
 
+class SortedTuning(ctypes.Structure):
+    """optphot_sorted_tuning: zero fields take the library defaults."""
+    _fields_ = [("slots_per_block", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("iterations_per_poll", ctypes.c_uint32), ("tail_threshold", ctypes.c_uint32)]
+
+
+class SortedStats(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint64), ("iterations", ctypes.c_uint64),
+                ("classified", ctypes.c_uint64), ("fresnel", ctypes.c_uint64),
+                ("rotated", ctypes.c_uint64), ("emitted", ctypes.c_uint64),
+                ("tail_photons", ctypes.c_uint64), ("loop_ms", ctypes.c_double),
+                ("tail_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class _RecordsC(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in RECORD_FIELDS]
 
@@ -112,6 +129,10 @@ def _load():
                                                            ctypes.c_uint64, P(WavefrontTuning),
                                                            P(EventStats), P(StepCounters),
                                                            P(RunTiming)]),
+        "optphot_run_gpu_tally_sorted": (ctypes.c_int, [P(SimParams), P(TallyConfig),
+                                                        ctypes.c_uint64, ctypes.c_void_p,
+                                                        ctypes.c_uint64, P(SortedTuning),
+                                                        P(SortedStats), P(RunTiming)]),
         "optphot_run_gpu_records_event": (ctypes.c_int, [P(SimParams), ctypes.c_uint64,
                                                          P(_RecordsC), ctypes.c_uint32,
                                                          ctypes.c_int, ctypes.c_uint64,
@@ -317,3 +338,21 @@ def run_gpu_tally_wavefront(p: SimParams, c: TallyConfig, n: int, first_id: int 
         st.launches = 1
         st.kernels_ms = tm.kernel_ms
     return Tally.from_flat(flat, c), tm, st, cnt
+
+
+def run_gpu_tally_sorted(p: SimParams, c: TallyConfig, n: int, first_id: int = 0,
+                         tuning: SortedTuning | dict | None = None):
+    """Tally by the per-interaction path (photon state in device memory; per
+    iteration a classify kernel (emission and the step's common part), then a
+    Fresnel and a rotation kernel over per-type queues; the design of the
+    reference's README.md:717-720).
+    Same tally as run_gpu_tally. Returns (Tally, RunTiming, SortedStats)."""
+    if isinstance(tuning, dict):
+        tuning = SortedTuning(**tuning)
+    flat = np.zeros(c.words, np.uint64)
+    tm, st = RunTiming(), SortedStats()
+    rc = _lib.optphot_run_gpu_tally_sorted(
+        ctypes.byref(p), ctypes.byref(c), n, flat.ctypes.data, first_id,
+        ctypes.byref(tuning) if tuning is not None else None, ctypes.byref(st), ctypes.byref(tm))
+    _check(rc)
+    return Tally.from_flat(flat, c), tm, st

@@ -177,14 +177,22 @@ struct Context {
   DevBuf gs_data, gs_slots;
   hipGraphExec_t step_exec = nullptr;
   std::string step_key;
+  // Per-interaction path: state, queues and rows in one buffer; the captured
+  // group of iterations; the pinned status words the host polls.
+  DevBuf sorted;
+  PinnedBuf sorted_status;
+  hipGraphExec_t sorted_exec = nullptr;
+  std::string sorted_key;
 
   // Every captured graph refers to the buffers it was built with.
   void drop_graphs() {
     if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
     if (step_exec) (void)hipGraphExecDestroy(step_exec);
-    batch_exec = step_exec = nullptr;
+    if (sorted_exec) (void)hipGraphExecDestroy(sorted_exec);
+    batch_exec = step_exec = sorted_exec = nullptr;
     batch_key.clear();
     step_key.clear();
+    sorted_key.clear();
   }
 
   void init() {
@@ -711,10 +719,8 @@ __attribute__((visibility("default"))) int optphot_valu_probe(uint32_t blocks, u
     Context& c = ctx();
     std::lock_guard<std::mutex> lock(c.mu);
     c.init();
-    double alloc = 0.0;
     // the probe's output lives in the partial-histogram buffer
-    const auto t0 = std::chrono::steady_clock::now();
-    if (c.partials.ensure(static_cast<size_t>(blocks) * 256 * sizeof(float))) alloc += ms_since(t0);
+    c.partials.ensure(static_cast<size_t>(blocks) * 256 * sizeof(float));
     HIPCHK(hipEventRecord(c.ev[0], c.stream));
     launch_valu_probe(c.partials.as<float>(), blocks, iters, half_lanes != 0, c.stream);
     HIPCHK(hipGetLastError());
@@ -872,6 +878,177 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records_event(
     st.issued_lane_steps = cnt.v[kCntIssued];
     tm.total_ms = ms_since(t0);
     if (stats) *stats = st;
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_tally_sorted(
+    const optphot_sim_params* p, const optphot_tally_config* tc, uint64_t n, uint64_t* tally,
+    uint64_t first_id, const optphot_sorted_tuning* tuning, optphot_sorted_stats* stats,
+    optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    if (!tc || !tally) throw InvalidArgument("tally config / output is NULL");
+    optphot_sorted_tuning tu{2048, 6, 8, 0};
+    if (tuning) {
+      if (tuning->slots_per_block) tu.slots_per_block = tuning->slots_per_block;
+      if (tuning->blocks_per_cu) tu.blocks_per_cu = tuning->blocks_per_cu;
+      if (tuning->iterations_per_poll) tu.iterations_per_poll = tuning->iterations_per_poll;
+      tu.tail_threshold = tuning->tail_threshold;
+    }
+    if (tu.slots_per_block % kSortedBlock != 0 || tu.slots_per_block > 65536)
+      throw InvalidArgument("sorted: slots_per_block must be a multiple of 256 in [256, 65536]");
+    if (tu.blocks_per_cu > 64)
+      throw InvalidArgument("sorted: blocks_per_cu must be in [1, 64]");
+    if (tu.iterations_per_poll % 2 != 0 || tu.iterations_per_poll > 1024)
+      throw InvalidArgument("sorted: iterations_per_poll must be even, in [2, 1024]");
+    if (n >= 0xffffffffull) throw InvalidArgument("sorted: n must be below 2^32 - 1; split the run");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    const auto t0 = std::chrono::steady_clock::now();
+    optphot_run_timing tm{};
+    const DevTally T = to_dev_tally(*tc, *p);
+    check_lds(c, T);
+    const uint32_t grid = tu.blocks_per_cu * static_cast<uint32_t>(c.cu_count);
+    const uint32_t seg = tu.slots_per_block;
+    const uint64_t W = static_cast<uint64_t>(grid) * seg;
+    if (W > (1ull << 29)) throw InvalidArgument("sorted: blocks x slots_per_block must be <= 2^29");
+    const uint64_t threshold = tu.tail_threshold ? tu.tail_threshold : W / 8;
+
+    // One buffer: 12 state arrays, 3 rotate + 2 fresnel queue arrays (all
+    // [W] x 4 B), then the per-block words and rows.
+    const size_t q_words = 17 * W;
+    const size_t off_cnt = q_words * 4;                               // 3 x [grid] u32
+    const size_t off_next = (off_cnt + 16ull * grid + 255) & ~255ull;  // [grid] u64
+    const size_t off_sums = off_next + 8ull * grid;                    // [2 grid] u64
+    const size_t off_stats = off_sums + 16ull * grid;                  // [grid][8] u64
+    const size_t off_status = off_stats + 8ull * kSortedStats * grid;  // [2] u64
+    const size_t off_rows = (off_status + 16 + 255) & ~255ull;          // [grid][words] u32
+    const size_t bytes = off_rows + 4ull * grid * T.words;
+    const auto ta = std::chrono::steady_clock::now();
+    bool grew = c.sorted.ensure(bytes);
+    grew |= c.tally.ensure((static_cast<size_t>(T.words) + 2) * 8);
+    grew |= c.sorted_status.ensure(16);
+    if (grew) {
+      tm.alloc_ms += ms_since(ta);
+      if (c.sorted_exec) (void)hipGraphExecDestroy(c.sorted_exec);
+      c.sorted_exec = nullptr;
+      c.sorted_key.clear();
+    }
+    char* base = c.sorted.as<char>();
+    uint32_t* q = c.sorted.as<uint32_t>();
+    SortedArgs A{};
+    float* f = reinterpret_cast<float*>(q);
+    A.px = f;
+    A.py = f + W;
+    A.pz = f + 2 * W;
+    A.dx = f + 3 * W;
+    A.dy = f + 4 * W;
+    A.dz = f + 5 * W;
+    A.t = f + 6 * W;
+    A.path = f + 7 * W;
+    A.nb = q + 8 * W;
+    A.ns = q + 9 * W;
+    A.idx = q + 10 * W;
+    A.draws = q + 11 * W;
+    A.rot_key = q + 12 * W;
+    A.rot_wc = q + 13 * W;
+    A.rot_wp = q + 14 * W;
+    A.fres_key = q + 15 * W;
+    A.fres_w = q + 16 * W;
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(base + off_cnt);
+    A.cnt_rot = cnt;
+    A.cnt_fres = cnt + grid;
+    A.occ = cnt + 2 * grid;
+    A.next_local = reinterpret_cast<unsigned long long*>(base + off_next);
+    A.sums = reinterpret_cast<unsigned long long*>(base + off_sums);
+    A.stats = reinterpret_cast<unsigned long long*>(base + off_stats);
+    A.status = reinterpret_cast<unsigned long long*>(base + off_status);
+    A.partials = reinterpret_cast<uint32_t*>(base + off_rows);
+    A.tally = T;
+    A.first_id = first_id;
+    A.n = n;
+    A.seg = seg;
+    A.grid = grid;
+    const DevParams P = to_dev(*p);
+    const PhiloxKey key = key_of(*p);
+
+    // The captured group: iterations_per_poll iterations, then the status.
+    std::string gkey(reinterpret_cast<const char*>(p), sizeof(*p));
+    gkey += "/" + std::to_string(n) + "/" + std::to_string(first_id) + "/" +
+            std::to_string(T.words) + "/" + std::to_string(T.time_scale) + "/" +
+            std::to_string(tc->xy_bins) + "/" + std::to_string(grid) + "/" + std::to_string(seg) +
+            "/" + std::to_string(tu.iterations_per_poll);
+    if (!c.sorted_exec || c.sorted_key != gkey) {
+      if (c.sorted_exec) HIPCHK(hipGraphExecDestroy(c.sorted_exec));
+      c.sorted_exec = nullptr;
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      for (uint32_t i = 0; i < tu.iterations_per_poll; ++i) launch_sorted_iteration(P, key, A, c.stream);
+      launch_sorted_status(A, c.stream);
+      HIPCHK(hipStreamEndCapture(c.stream, &g));
+      HIPCHK(hipGraphInstantiate(&c.sorted_exec, g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+      c.sorted_key = gkey;
+    }
+
+    // Every photon takes at most max_steps steps and each occupied slot takes
+    // one step per iteration, so the loop ends within this many iterations.
+    const uint64_t per_block = (n + grid - 1) / grid;
+    const uint64_t bound =
+        ((per_block + seg - 1) / seg + 1) * (static_cast<uint64_t>(p->max_steps) + 2) +
+        2ull * tu.iterations_per_poll;
+    auto* st = static_cast<volatile unsigned long long*>(c.sorted_status.ptr);
+    HIPCHK(hipEventRecord(c.ev[0], c.stream));
+    launch_sorted_init(A, c.stream);
+    HIPCHK(hipGetLastError());
+    uint64_t iterations = 0;
+    for (;;) {
+      HIPCHK(hipGraphLaunch(c.sorted_exec, c.stream));
+      iterations += tu.iterations_per_poll;
+      HIPCHK(hipMemcpyAsync(c.sorted_status.ptr, A.status, 16, hipMemcpyDeviceToHost, c.stream));
+      HIPCHK(hipStreamSynchronize(c.stream));
+      const uint64_t occupied = st[0], left = st[1];
+      if (left == 0 && (occupied <= threshold || occupied == 0)) break;
+      if (iterations > bound)
+        throw std::runtime_error("sorted: photons still in flight after " +
+                                 std::to_string(iterations) + " iterations");
+    }
+    HIPCHK(hipEventRecord(c.ev[1], c.stream));
+    launch_sorted_tail(P, key, A, c.stream);
+    HIPCHK(hipGetLastError());
+    launch_zero_u64(c.tally.as<unsigned long long>(), static_cast<size_t>(T.words) + 2, c.stream);
+    launch_tally_reduce(T, static_cast<int>(grid), A.partials, A.sums,
+                        c.tally.as<unsigned long long>(), c.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev[2], c.stream));
+    const float loop_ms = c.elapsed(0, 1);
+    tm.kernel_ms = c.elapsed(0, 2);
+
+    const auto td = std::chrono::steady_clock::now();
+    HIPCHK(hipMemcpyAsync(tally, c.tally.ptr, (static_cast<size_t>(T.words) + 2) * 8,
+                          hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    tm.d2h_ms = ms_since(td);
+    if (stats) {
+      std::vector<unsigned long long> rows(static_cast<size_t>(grid) * kSortedStats);
+      HIPCHK(hipMemcpy(rows.data(), A.stats, rows.size() * 8, hipMemcpyDeviceToHost));
+      unsigned long long sum[kSortedStats] = {};
+      for (size_t i = 0; i < rows.size(); ++i) sum[i % kSortedStats] += rows[i];
+      *stats = optphot_sorted_stats{};
+      stats->slots = W;
+      stats->iterations = iterations;
+      stats->classified = sum[kSortedClassified];
+      stats->fresnel = sum[kSortedFresnel];
+      stats->rotated = sum[kSortedRotate];
+      stats->emitted = sum[kSortedEmitted];
+      stats->tail_photons = sum[kSortedTail];
+      stats->loop_ms = loop_ms;
+      stats->tail_ms = tm.kernel_ms - loop_ms;
+    }
+    tm.h2d_ms = 0.0;
+    tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
   });
 }

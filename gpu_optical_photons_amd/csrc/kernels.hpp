@@ -116,6 +116,61 @@ inline uint32_t slots_per_claim(uint64_t n_slots, uint64_t waves) {
   return static_cast<uint32_t>(k < 1u ? 1u : (k > 16u ? 16u : k));
 }
 
+// Per-interaction ("sorted") wavefront launches. Photon state lives in HBM
+// as structure-of-arrays over W = grid * seg slots; block b of every kernel
+// owns slots [b * seg, (b + 1) * seg) and photon indices [lo_b, hi_b) with
+// lo_b = n * b / grid, so every queue below is a set of per-block segments
+// whose counters only block b touches (no contended atomics). One iteration
+// is three launches:
+//   classify: for every slot, in slot order (coalesced): an empty slot takes
+//             the block's next photon (emission); then one step's common
+//             part; the photon then goes on one of the per-type queues below,
+//             stays ready (specular reflection), or finishes;
+//   fresnel:  readout and polished-face decisions from their queue;
+//   rotate:   Rayleigh and Lambertian directions from their queue.
+struct SortedArgs {
+  // photon state [W]; idx = photon index - first_id, kSortedIdle if the
+  // slot holds no photon
+  float *px, *py, *pz, *dx, *dy, *dz, *t, *path;
+  uint32_t *nb, *ns, *idx, *draws;
+  // queues: entries [W] (segment b at b * seg), counts [grid]
+  uint32_t *rot_key, *rot_wc, *rot_wp, *cnt_rot;  // key = slot | code << 29
+  uint32_t *fres_key, *fres_w, *cnt_fres;         // key = slot | face << 29
+  unsigned long long* next_local;  // [grid] next photon index of block b
+  uint32_t* occ;                   // [grid] occupied slots of block b
+  uint32_t* partials;              // [grid][tally.words]
+  unsigned long long* sums;        // [grid][2] boundary / scatter sums
+  unsigned long long* stats;       // [grid][kSortedStats]
+  unsigned long long* status;      // [2]: occupied slots, blocks with photons left
+  DevTally tally;
+  uint64_t first_id, n;
+  uint32_t seg, grid;
+};
+
+constexpr uint32_t kSortedIdle = 0xffffffffu;
+
+enum : int {
+  kSortedClassified = 0,  // photon-steps of classify
+  kSortedFresnel = 1,
+  kSortedRotate = 2,
+  kSortedEmitted = 3,
+  kSortedTail = 4,  // photons finished by the tail kernel
+  kSortedStats = 8
+};
+
+constexpr int kSortedBlock = 256;
+
+// Empties every slot; resets the counters, partial rows and stats.
+void launch_sorted_init(const SortedArgs& A, hipStream_t stream);
+// One iteration: classify, fresnel, rotate.
+void launch_sorted_iteration(const DevParams& P, PhiloxKey key, const SortedArgs& A,
+                             hipStream_t stream);
+// status[0] = occupied slots, status[1] = blocks with photon indices left.
+void launch_sorted_status(const SortedArgs& A, hipStream_t stream);
+// Finishes the photons in the occupied slots, one lane per photon (history style).
+void launch_sorted_tail(const DevParams& P, PhiloxKey key, const SortedArgs& A,
+                        hipStream_t stream);
+
 // History-based launches: one lane per photon (records), or a grid-stride
 // loop over photons (tally). first_id offsets the Philox photon ids; records
 // are written at index i - first_id.

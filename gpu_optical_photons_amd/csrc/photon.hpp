@@ -505,23 +505,32 @@ __device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenste
   ph.ns = 0u;
 }
 
-// One transport step (transport.hpp:71-145). Returns true while the photon
-// is alive; otherwise *fate is set. The stream must hold the step's (up to 5)
-// words: Stream after refill() (pos <= 3), RingStream with level() >= 5.
-// kSurf >= 0 fixes the surface model at compile time (a launch whose events
-// all share it); kSurf < 0 reads it from P.
-template <int kSurf = -1, bool kAdj = false, class RNG>
-__device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, uint32_t& fate,
-                                     StepInfo& info, AdjState* adj = nullptr) {
-  const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
-  // --- two exponential distances, always drawn (transport.hpp:76-77) -------
+// The phases of one transport step (transport.hpp:71-145), shared by the
+// fused step() below and the per-interaction kernels, which run them in
+// separate launches on compacted queues.
+
+// Where the step ends: absorbed in the bulk, scattered, or on face `face`
+// (axis = face / 2, plus = odd face); cos_i = dot(d, outward normal) at a
+// boundary.
+struct Advance {
+  bool absorb, scatter, boundary, plus;
+  int face, axis;
+  float cos_i;
+};
+
+// Phase 1: the two exponential distances (draws 0 and 1, always drawn,
+// transport.hpp:76-77), the distance to the wall approached (geometry.hpp:
+// 25-46), which event wins, and one shared advance with the snap onto a face
+// (geometry.hpp:57-63) and the boundary / scatter counts.
+template <class RNG>
+__device__ __forceinline__ Advance advance(const DevParams& P, const RNG& rng, Photon& ph) {
   const float u_abs = to_unit(rng.template peek<0>());
   const float u_sca = to_unit(rng.template peek<1>());
   float s_abs = kInf, s_sca = kInf;
   if (P.abs_length != kInf) s_abs = -P.abs_length * p_log(u_abs);
   if (P.scat_length != kInf) s_sca = -P.scat_length * p_log(u_sca);
 
-  // --- distance to the wall approached, ties to the lower axis (geometry.hpp:25-46)
+  // --- distance to the wall approached, ties to the lower axis --------------
   float dist = kInf;
   int face = -1;
   if (ph.d.x != 0.0f) {
@@ -551,58 +560,105 @@ __device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, u
   dist = __builtin_fmaxf(dist, 0.0f);
 
   // --- which event wins, and one shared advance ------------------------------
-  const bool absorb = s_abs < dist && s_abs <= s_sca;
-  const bool scatter = !absorb && s_sca < dist;
-  const bool boundary = !absorb && !scatter;
-  const float go = absorb ? s_abs : (scatter ? s_sca : dist);
+  Advance a;
+  a.absorb = s_abs < dist && s_abs <= s_sca;
+  a.scatter = !a.absorb && s_sca < dist;
+  a.boundary = !a.absorb && !a.scatter;
+  const float go = a.absorb ? s_abs : (a.scatter ? s_sca : dist);
   ph.p = V3{ph.p.x + go * ph.d.x, ph.p.y + go * ph.d.y, ph.p.z + go * ph.d.z};
   ph.t += go * P.inv_speed;
   ph.path += go;
 
-  const int axis = face >> 1;
-  const bool plus = face & 1;
-  if (boundary) {  // snap onto the face plane, clamp the others (geometry.hpp:57-63)
-    const float s = plus ? 1.0f : -1.0f;
-    ph.p.x = axis == 0 ? s * P.hx : __builtin_fminf(__builtin_fmaxf(ph.p.x, -P.hx), P.hx);
-    ph.p.y = axis == 1 ? s * P.hy : __builtin_fminf(__builtin_fmaxf(ph.p.y, -P.hy), P.hy);
-    ph.p.z = axis == 2 ? s * P.hz : __builtin_fminf(__builtin_fmaxf(ph.p.z, -P.hz), P.hz);
+  a.face = face;
+  a.axis = face >> 1;
+  a.plus = face & 1;
+  if (a.boundary) {  // snap onto the face plane, clamp the others
+    const float s = a.plus ? 1.0f : -1.0f;
+    ph.p.x = a.axis == 0 ? s * P.hx : __builtin_fminf(__builtin_fmaxf(ph.p.x, -P.hx), P.hx);
+    ph.p.y = a.axis == 1 ? s * P.hy : __builtin_fminf(__builtin_fmaxf(ph.p.y, -P.hy), P.hy);
+    ph.p.z = a.axis == 2 ? s * P.hz : __builtin_fminf(__builtin_fmaxf(ph.p.z, -P.hz), P.hz);
     ++ph.nb;
   }
-  if (scatter) ++ph.ns;
+  if (a.scatter) ++ph.ns;
 
   // cos_i = dot(d, outward normal) is exactly +-d[axis] (the zero components
   // contribute signed zeros that vanish in the sum).
-  const float d_axis = axis == 0 ? ph.d.x : (axis == 1 ? ph.d.y : ph.d.z);
-  const float cos_i = plus ? d_axis : -d_axis;
+  const float d_axis = a.axis == 0 ? ph.d.x : (a.axis == 1 ? ph.d.y : ph.d.z);
+  a.cos_i = a.plus ? d_axis : -d_axis;
+  return a;
+}
 
-  const bool readout = boundary && face == kReadoutFace;
-  const bool side = boundary && face != kReadoutFace;
+// Phase 2 (readout and polished faces): the Fresnel reflection probability
+// (fresnel.hpp:30-46) from the scintillator into the detector (readout) or
+// the outside medium. `scored` is set when the decision carries an adjoint
+// score: matched interfaces (R = 0 to second order) and total internal
+// reflection (R = 1 on an open set of angles) carry none.
+__device__ __forceinline__ float fresnel_prob(const DevParams& P, bool readout, float cos_i,
+                                              bool& scored) {
+  const float n1 = P.n_scint;
+  const float n2 = readout ? P.n_det : P.n_out;
+  const float eta2 = readout ? P.eta2_det : P.eta2_out;
+  const bool matched = readout ? P.det_matched : P.out_matched;
+  const float sin2_t = eta2 * __builtin_fmaxf(0.0f, 1.0f - cos_i * cos_i);
+  // 1 - sin2_t is in (0, 1] and at least 2^-24 unless total internal
+  // reflection (sin2_t >= 1), where R is not used
+  const float cos_t = sqrt_n(1.0f - sin2_t);
+  const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
+  const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
+  const float R = 0.5f * (rs * rs + rp * rp);
+  const bool tir = sin2_t >= 1.0f;
+  scored = !matched && !tir;
+  return matched ? 0.0f : (tir ? 1.0f : R);
+}
+
+// Specular reflection on an axis-aligned face: d - 2 (d.n) n rounds to
+// exactly the negation of the approached component.
+__device__ __forceinline__ void flip_axis(V3& d, int axis) {
+  if (axis == 0) d.x = -d.x;
+  if (axis == 1) d.y = -d.y;
+  if (axis == 2) d.z = -d.z;
+}
+
+// Phase 3: the new direction after Rayleigh scattering (about the direction,
+// cos from Cardano, sampling.hpp:39-49) or Lambertian reflection (about the
+// inward normal -n of the face on `axis`, cos = sqrt(u), sampling.hpp:53-55),
+// from the two words wc (cos) and wp (azimuth).
+__device__ __forceinline__ V3 redirect(bool scatter, V3 d, int axis, bool plus, uint32_t wc,
+                                       uint32_t wp) {
+  const float uc = to_unit(wc), up = to_unit(wp);
+  float cos_theta;
+  if (scatter)
+    cos_theta = rayleigh_cos(uc);
+  else
+    cos_theta = sqrt_n(uc);  // uc in [2^-24, 1]
+  const float s = plus ? 1.0f : -1.0f;
+  // -n keeps the reference's signed zeros: -(+0) = -0 off-axis.
+  const V3 inward{axis == 0 ? -s : -0.0f, axis == 1 ? -s : -0.0f, axis == 2 ? -s : -0.0f};
+  return rotate_about(scatter ? d : inward, cos_theta, up);
+}
+
+// One transport step (transport.hpp:71-145). Returns true while the photon
+// is alive; otherwise *fate is set. The stream must hold the step's (up to 5)
+// words: Stream after refill() (pos <= 3), RingStream with level() >= 5.
+// kSurf >= 0 fixes the surface model at compile time (a launch whose events
+// all share it); kSurf < 0 reads it from P.
+template <int kSurf = -1, bool kAdj = false, class RNG>
+__device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, uint32_t& fate,
+                                     StepInfo& info, AdjState* adj = nullptr) {
+  const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
+  const Advance a = advance(P, rng, ph);
+  const bool absorb = a.absorb, scatter = a.scatter;
+  const bool readout = a.boundary && a.face == kReadoutFace;
+  const bool side = a.boundary && a.face != kReadoutFace;
   const bool polished = side && surface == kPolished;
   const bool black = side && surface == kBlack;
   const bool refl = side && surface >= kSpecular;
   const bool lambert = surface == kLambertian;
 
-  // --- Fresnel (fresnel.hpp:30-46), once for readout and polished lanes -----
+  // --- Fresnel, once for readout and polished lanes --------------------------
   float prob = P.reflectivity;
   bool scored = false;  // a Fresnel decision with a score (adjoint only)
-  if (readout || polished) {
-    const float n1 = P.n_scint;
-    const float n2 = readout ? P.n_det : P.n_out;
-    const float eta2 = readout ? P.eta2_det : P.eta2_out;
-    const bool matched = readout ? P.det_matched : P.out_matched;
-    const float sin2_t = eta2 * __builtin_fmaxf(0.0f, 1.0f - cos_i * cos_i);
-    // 1 - sin2_t is in (0, 1] and at least 2^-24 unless total internal
-    // reflection (sin2_t >= 1), where R is not used
-    const float cos_t = sqrt_n(1.0f - sin2_t);
-    const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
-    const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
-    const float R = 0.5f * (rs * rs + rp * rp);
-    const bool tir = sin2_t >= 1.0f;
-    prob = matched ? 0.0f : (tir ? 1.0f : R);
-    // Matched interfaces (R = 0 to second order) and total internal
-    // reflection (R = 1 on an open set of angles) carry no score.
-    if (kAdj) scored = !matched && !tir;
-  }
+  if (readout || polished) prob = fresnel_prob(P, readout, a.cos_i, scored);
   // --- the surface decision draw (u <= prob survives, transport.hpp:41) -----
   const bool decide = readout || polished || refl;
   const bool survived = decide && to_unit(rng.template peek<2>()) <= prob;
@@ -611,7 +667,7 @@ __device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, u
     // Bernoulli score: d log R (reflected) or d log (1 - R) (transmitted);
     // readout transmissions are skipped (their baseline term is zero).
     float d1, d2;
-    fresnel_score(P.n_scint, readout ? P.n_det : P.n_out, cos_i, survived, d1, d2);
+    fresnel_score(P.n_scint, readout ? P.n_det : P.n_out, a.cos_i, survived, d1, d2);
     adj->s_nscint += d1;
     if (readout) {
       adj->s_ndet += d2;
@@ -623,26 +679,12 @@ __device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, u
   if (kAdj && refl) ++adj->refl_hits;
 
   // --- new direction ----------------------------------------------------------
-  if (survived && !lambert_reflect) {  // specular: flip the approached component
-    if (axis == 0) ph.d.x = -ph.d.x;
-    if (axis == 1) ph.d.y = -ph.d.y;
-    if (axis == 2) ph.d.z = -ph.d.z;
-  }
+  if (survived && !lambert_reflect) flip_axis(ph.d, a.axis);
   if (scatter || lambert_reflect) {
-    // Rayleigh: about the direction, cos from Cardano, draws 3, 4.
-    // Lambertian: about the inward normal -n, cos = sqrt(u), draws 4, 5.
+    // Rayleigh: draws 2, 3. Lambertian: draws 3, 4.
     const uint32_t wc = scatter ? rng.template peek<2>() : rng.template peek<3>();
     const uint32_t wp = scatter ? rng.template peek<3>() : rng.template peek<4>();
-    const float uc = to_unit(wc), up = to_unit(wp);
-    float cos_theta;
-    if (scatter)
-      cos_theta = rayleigh_cos(uc);
-    else
-      cos_theta = sqrt_n(uc);  // uc in [2^-24, 1]
-    const float s = plus ? 1.0f : -1.0f;
-    // -n keeps the reference's signed zeros: -(+0) = -0 off-axis.
-    const V3 inward{axis == 0 ? -s : -0.0f, axis == 1 ? -s : -0.0f, axis == 2 ? -s : -0.0f};
-    ph.d = rotate_about(scatter ? ph.d : inward, cos_theta, up);
+    ph.d = redirect(scatter, ph.d, a.axis, a.plus, wc, wp);
   }
   rng.pos += 2u + (decide ? 1u : 0u) + ((scatter || lambert_reflect) ? 2u : 0u);
 

@@ -218,6 +218,14 @@ void test_gpu() {
         th.xy_hist == tw.xy_hist && th.boundary_sum == tw.boundary_sum &&
         th.scatter_sum == tw.scatter_sum);
   CHECK(th.total() == n);
+  // the per-interaction path gives the same tally
+  Tally ts;
+  SortedStats ss{};
+  run_gpu_tally_sorted(p, c, n, ts, 7, nullptr, &ss);
+  CHECK(th.fate_counts == ts.fate_counts && th.time_hist == ts.time_hist &&
+        th.xy_hist == ts.xy_hist && th.boundary_sum == ts.boundary_sum &&
+        th.scatter_sum == ts.scatter_sum);
+  CHECK(ss.emitted == n && ss.iterations >= 2);
   uint64_t det = 0, nb = 0;
   for (std::size_t i = 0; i < n; ++i) {
     det += h.fate[i] == 0;

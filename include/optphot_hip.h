@@ -189,6 +189,46 @@ int optphot_run_gpu_tally_counters(const optphot_sim_params* p, const optphot_ta
                                    uint64_t n, uint64_t* tally, int block_size, uint64_t first_id,
                                    optphot_step_counters* counters, optphot_run_timing* timing);
 
+/* Per-interaction ("sorted") wavefront path: photon state in device memory,
+ * and per iteration one kernel for emission into empty slots and the common
+ * part of a step (which sorts the photons onto per-interaction queues), one
+ * for the readout / polished-face Fresnel decisions and one for the
+ * Rayleigh / Lambertian directions. This is the design the
+ * reference lists as its next step ("per-interaction kernels with sorting by
+ * event type", README.md:717-720). Zero fields take the defaults. */
+typedef struct optphot_sorted_tuning {
+  uint32_t slots_per_block;     /* photons in flight per block: a multiple of 256
+                                   in [256, 65536]; default 2048 */
+  uint32_t blocks_per_cu;       /* blocks = blocks_per_cu x CUs, in [1, 64];
+                                   default 6 (the classify kernel's occupancy) */
+  uint32_t iterations_per_poll; /* iterations per graph replay between host
+                                   checks: even, in [2, 1024]; default 8 */
+  uint32_t tail_threshold;      /* once every photon is emitted and at most
+                                   this many are in flight, one kernel
+                                   finishes them history-style; default
+                                   slots / 8 (0xffffffff: at once) */
+} optphot_sorted_tuning;
+
+typedef struct optphot_sorted_stats {
+  uint64_t slots;        /* photons in flight at most (blocks x slots_per_block) */
+  uint64_t iterations;   /* classify / fresnel / rotate rounds run */
+  uint64_t classified;   /* photon-steps of the classify kernel */
+  uint64_t fresnel;      /* Fresnel decisions */
+  uint64_t rotated;      /* new directions (Rayleigh + Lambertian) */
+  uint64_t emitted;      /* photons emitted */
+  uint64_t tail_photons; /* photons finished by the tail kernel */
+  double loop_ms;        /* init + iterations, host checks included */
+  double tail_ms;        /* tail kernel and tally reduction */
+} optphot_sorted_stats;
+
+/* Tally of photons first_id .. first_id+n-1 (n < 2^32 - 1) by the
+ * per-interaction path. Same tally as optphot_run_gpu_tally, word for word.
+ * tuning / stats / timing may be NULL. */
+int optphot_run_gpu_tally_sorted(const optphot_sim_params* p, const optphot_tally_config* c,
+                                 uint64_t n, uint64_t* tally, uint64_t first_id,
+                                 const optphot_sorted_tuning* tuning, optphot_sorted_stats* stats,
+                                 optphot_run_timing* timing);
+
 /* Batched events. Event e has parameters params[e] and photons[e] photons,
  * with Philox photon ids first_id + offset_e .. first_id + offset_e +
  * photons[e] - 1, offset_e = photons[0] + ... + photons[e-1]; every event

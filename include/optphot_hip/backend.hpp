@@ -203,6 +203,23 @@ inline RunTiming run_gpu_tally_wavefront(const SimParams& p, const TallyConfig& 
   return tm;
 }
 
+// The same tally from the per-interaction path (per-type queues and one
+// kernel per interaction; the design of the reference's README.md:717-720).
+using SortedTuning = optphot_sorted_tuning;
+using SortedStats = optphot_sorted_stats;
+inline RunTiming run_gpu_tally_sorted(const SimParams& p, const TallyConfig& c, uint64_t n,
+                                      Tally& out, uint64_t first_id = 0,
+                                      const SortedTuning* tuning = nullptr,
+                                      SortedStats* stats = nullptr) {
+  std::vector<uint64_t> w(detail::tally_words(c));
+  RunTiming tm;
+  detail::check(optphot_run_gpu_tally_sorted(c_params(p), c_tally_config(c), n, w.data(),
+                                             first_id, tuning, stats,
+                                             reinterpret_cast<optphot_run_timing*>(&tm)));
+  detail::unpack(w, c, out);
+  return tm;
+}
+
 // Records written straight into caller-owned device arrays (n elements each,
 // e.g. hipMalloc'ed or a framework's tensors): no copy to the host.
 // steps_per_launch 0 = history kernel, >= 1 = wavefront kernel.
