@@ -123,6 +123,42 @@ def bench_gensteps():
             "photons_per_s": n / (min(ms) * 1e-3), "upload_ms": tm.h2d_ms}
 
 
+def bench_sorted(p, c, n, fused_tally, counters, ring_tuning):
+    """Sorting by interaction type, two forms, on the long-histories workload
+    (same photons as the headline line):
+      * the per-interaction path (optphot_run_gpu_tally_sorted): photon state
+        in HBM, per iteration a classify kernel (emission + the step's common
+        part, sorting photons onto per-block queues), a Fresnel kernel and a
+        rotation kernel; bound by HBM traffic, so its roofline is bytes;
+      * the ring kernel with each step's Fresnel and rotation work sorted onto
+        block-wide LDS queues (wavefront tuning sorted_phases=1).
+    Both must reproduce the headline tally word for word."""
+    from gpu_optical_photons_amd import backend, roofline
+    backend.run_gpu_tally_sorted(p, c, n)  # warm-up (and graph capture)
+    ms, st, t = [], None, None
+    for _ in range(3):
+        t, tm, st = backend.run_gpu_tally_sorted(p, c, n)
+        ms.append(tm.kernel_ms)
+    best = min(ms)
+    b = roofline.sorted_path_bytes(counters, n, st.slots, st.iterations,
+                                   int(fused_tally.fate_counts[0]))
+    out = {"path": {"kernel_ms": best, "photons_per_s": n / (best * 1e-3),
+                    "identical_to_fused": bool((t.flat() == fused_tally.flat()).all()),
+                    "stats": st.as_dict(), "algorithmic_bytes": b,
+                    "achieved_GBps": b / (best * 1e-3) / 1e9,
+                    "hbm_frac": b / (best * 1e-3) / roofline.HBM_PEAK_BYTES_PER_S}}
+    tu = dict(ring_tuning, sorted_phases=1)
+    backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)
+    ms = []
+    for _ in range(3):
+        t2, tm, _, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)
+        ms.append(tm.kernel_ms)
+    out["lds_phases"] = {"tuning": tu, "kernel_ms": min(ms),
+                         "photons_per_s": n / (min(ms) * 1e-3),
+                         "identical_to_fused": bool((t2.flat() == fused_tally.flat()).all())}
+    return out
+
+
 def bench_batched():
     """Batched events: 4096 events (box ~U[10,100] mm per axis, surface model
     uniform over polished / black / specular / Lambertian, n_det ~U[1.4,1.6],
@@ -210,6 +246,9 @@ def main():
     ap.add_argument("--photons", type=int, default=N_PHOTONS)
     ap.add_argument("--only-main", action="store_true",
                     help="only the long-histories line (no other workloads)")
+    ap.add_argument("--sorted-only", action="store_true",
+                    help="only the sorting-by-interaction-type measurements on the "
+                         "long-histories workload")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the library's VALU calibration kernel (all lanes, then odd "
                          "lanes only): a known instruction count for reading rocprofv3 VALU "
@@ -269,10 +308,17 @@ def main():
     c = TallyConfig(256, 100.0, 64)
     n = args.photons
     tuning = autotune.choice_for("wavefront_tally", p, c)
-    kernel_name = ("wavefront_ring_kernel" if tuning.get("stream_in_lds")
+    kernel_name = ("wavefront_phase_kernel" if tuning.get("sorted_phases")
+                   else "wavefront_ring_kernel" if tuning.get("stream_in_lds")
                    else "wavefront_next_kernel" if tuning.get("successor_threshold")
                    else "wavefront_kernel")
 
+    if args.sorted_only:
+        ring = autotune.choice_for("wavefront_tally", p, c)
+        fused, _, _, cnt = backend.run_gpu_tally_wavefront(p, c, n, tuning=ring, counters=True)
+        print(json.dumps({"workload": WORKLOAD, "photons": n,
+                          "sorted": bench_sorted(p, c, n, fused, cnt.as_dict(), ring)}))
+        return
     for _ in range(args.warmup):
         backend.run_gpu_tally_wavefront(p, c, n, tuning=tuning, counters=False)
     kernel_ms, transport_ms = [], []
@@ -360,6 +406,7 @@ def main():
         line["gensteps"] = bench_gensteps()
         line["differentiation"] = bench_differentiation()
         line["calibration"] = bench_calibration()
+        line["sorted_by_interaction"] = bench_sorted(p, c, n, tally, counters, tuning)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, c)
     print(json.dumps(line), flush=True)

@@ -24,13 +24,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuning.json")
 
 DEFAULT = {"block_size": 256, "steps_per_refill": 4, "queue_chunk": 256, "blocks_per_cu": 0,
-           "successor_threshold": 0, "stream_in_lds": 0}
+           "successor_threshold": 0, "stream_in_lds": 0, "sorted_phases": 0}
 
 BLOCKS = (64, 128, 256, 512, 1024)
 STEPS = (1, 2, 4, 8, 16)
 THRESHOLDS = (16, 32, 48, 64)
 RING_BLOCKS = (128, 256, 512)
 RING_STEPS = (1, 2, 3, 4)
+PHASE_BLOCKS = (256, 512, 1024)
+PHASE_STEPS = (1, 2)
 CHUNKS = (128, 512, 2048)
 
 
@@ -59,7 +61,8 @@ def choice_for(kernel: str, p, c=None) -> dict:
 
 
 def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
-          thresholds=THRESHOLDS, ring_blocks=RING_BLOCKS, ring_steps=RING_STEPS, reps: int = 2,
+          thresholds=THRESHOLDS, ring_blocks=RING_BLOCKS, ring_steps=RING_STEPS,
+          phase_blocks=PHASE_BLOCKS, phase_steps=PHASE_STEPS, reps: int = 2,
           save: bool = True, log=print) -> dict:
     """Times every candidate on n photons; keeps the fastest. Every candidate
     must give the same tally word for word."""
@@ -71,15 +74,17 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
         raise ValueError(f"unknown kernel {kernel!r}")
     ref = None
     results = []
-    candidates = [{"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0,
-                   "successor_threshold": 0, "stream_in_lds": 0}
+    candidates = [DEFAULT | {"block_size": b, "steps_per_refill": k, "queue_chunk": ch}
                   for b, k, ch in itertools.product(blocks, steps, chunks)]
-    candidates += [{"block_size": b, "steps_per_refill": 1, "queue_chunk": ch, "blocks_per_cu": 0,
-                    "successor_threshold": th, "stream_in_lds": 0}
+    candidates += [DEFAULT | {"block_size": b, "steps_per_refill": 1, "queue_chunk": ch,
+                              "successor_threshold": th}
                    for b, th, ch in itertools.product(blocks, thresholds, chunks)]
-    candidates += [{"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "blocks_per_cu": 0,
-                    "successor_threshold": 0, "stream_in_lds": 1}
+    candidates += [DEFAULT | {"block_size": b, "steps_per_refill": k, "queue_chunk": ch,
+                              "stream_in_lds": 1}
                    for b, k, ch in itertools.product(ring_blocks, ring_steps, chunks)]
+    candidates += [DEFAULT | {"block_size": b, "steps_per_refill": k, "queue_chunk": ch,
+                              "sorted_phases": 1}
+                   for b, k, ch in itertools.product(phase_blocks, phase_steps, chunks)]
     for tu in candidates:
         best = math.inf
         try:
@@ -96,7 +101,8 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
             raise AssertionError(f"tally differs for tuning {tu}: the kernel is not "
                                  "schedule-independent")
         results.append((best, tu))
-        policy = (f"ring steps={tu['steps_per_refill']:2d}" if tu["stream_in_lds"]
+        policy = (f"phase steps={tu['steps_per_refill']:2d}" if tu["sorted_phases"]
+                  else f"ring steps={tu['steps_per_refill']:2d}" if tu["stream_in_lds"]
                   else f"successor>={tu['successor_threshold']:2d}" if tu["successor_threshold"]
                   else f"steps={tu['steps_per_refill']:2d}")
         log(f"  {kernel} block={tu['block_size']:4d} {policy:14s} chunk={tu['queue_chunk']:5d}: "

@@ -60,7 +60,8 @@ class StepCounters(ctypes.Structure):
 class WavefrontTuning(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_int), ("steps_per_refill", ctypes.c_uint32),
                 ("queue_chunk", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_int),
-                ("successor_threshold", ctypes.c_uint32), ("stream_in_lds", ctypes.c_uint32)]
+                ("successor_threshold", ctypes.c_uint32), ("stream_in_lds", ctypes.c_uint32),
+                ("sorted_phases", ctypes.c_uint32)]
 
 
 class SortedTuning(ctypes.Structure):

@@ -427,6 +427,7 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
     if (tally && !records) {
       t.successor_threshold = tu->successor_threshold;
       t.stream_in_lds = tu->stream_in_lds;
+      t.sorted_phases = tu->sorted_phases;
     }
   }
   if (t.successor_threshold > 64)
@@ -434,6 +435,10 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   if (t.stream_in_lds && ring_lds_bytes(static_cast<uint32_t>(lds / 4), t.block_size) > c.max_lds)
     throw InvalidArgument("wavefront: tally plus LDS streams exceed the LDS of a block; use a "
                           "smaller block_size");
+  if (t.sorted_phases &&
+      phase_lds_bytes(static_cast<uint32_t>(lds / 4), t.block_size) > c.max_lds)
+    throw InvalidArgument("wavefront: tally, LDS streams and phase queues exceed the LDS of a "
+                          "block; use a smaller block_size");
   validate_block(t.block_size, "wavefront");
   if (t.queue_chunk < 64 || t.queue_chunk % 64 != 0 || t.queue_chunk > (1u << 24))
     throw InvalidArgument("wavefront: queue_chunk must be a multiple of 64 in [64, 2^24]");
@@ -441,7 +446,8 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
     throw InvalidArgument("wavefront: steps_per_refill must be >= 1");
   const int max_per_cu =
       std::max(1, wavefront_blocks_per_cu(records, tally, t.block_size, lds,
-                                          t.successor_threshold > 0, t.stream_in_lds != 0));
+                                          t.successor_threshold > 0, t.stream_in_lds != 0,
+                                          t.sorted_phases != 0));
   const int per_cu = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, max_per_cu) : max_per_cu;
   const uint64_t needed = (n + t.block_size - 1) / t.block_size;
   WavefrontShape s{};
@@ -452,6 +458,7 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   s.queue_chunk = t.queue_chunk;
   s.successor_threshold = t.successor_threshold;
   s.stream_in_lds = t.stream_in_lds;
+  s.sorted_phases = t.sorted_phases;
   return s;
 }
 

@@ -106,6 +106,9 @@ struct WavefrontShape {
   uint32_t stream_in_lds = 0;
   // Batch / genstep launches: queue slots a wave claims per atomic.
   uint32_t slots_per_claim = 1;
+  // Tally launches: ring streams plus each step's Fresnel and rotation work
+  // sorted onto block-wide LDS queues (takes precedence over both above).
+  uint32_t sorted_phases = 0;
 };
 
 // Slots per claim for a slot queue of n_slots drained by `waves` waves: up to
@@ -251,7 +254,10 @@ int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);
 // LDS bytes of a wavefront tally launch with the streams in LDS: the tally
 // words, then 16 ring words per lane.
 size_t ring_lds_bytes(uint32_t tally_words, int block);
+// LDS bytes of the sorted-phase kernel: the ring layout, then the per-parity
+// queue counters and 9 words per lane of queues and exchange slots.
+size_t phase_lds_bytes(uint32_t tally_words, int block);
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds,
-                            bool successor = false, bool ring = false);
+                            bool successor = false, bool ring = false, bool phases = false);
 
 }  // namespace ophip

@@ -38,18 +38,6 @@ constexpr uint32_t kSlotBits = 29;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1u;
 constexpr uint32_t kCodeScatter = 6;  // rotate queue: Rayleigh; codes 0..5: Lambertian on that face
 
-// Entry of a block-local queue for the lanes with `want`: one LDS atomic per
-// wave. Called by all lanes of the wave.
-__device__ __forceinline__ uint32_t reserve(uint32_t* counter, bool want) {
-  const uint64_t m = __ballot(want);
-  uint32_t base = 0;
-  if (m != 0) {
-    if (lane_id() == 0) base = atomicAdd(counter, popc(m));
-    base = __builtin_amdgcn_readfirstlane(base);
-  }
-  return base + rank_in(m);
-}
-
 __device__ __forceinline__ uint64_t block_lo(const SortedArgs& A, uint32_t b) {
   return A.n * b / A.grid;
 }

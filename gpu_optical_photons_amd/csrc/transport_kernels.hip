@@ -540,6 +540,276 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
 }
 
 // ---------------------------------------------------------------------------
+// Persistent wavefront kernel with each step's branch work sorted by
+// interaction type (tally). The ring kernel above runs the Fresnel decision
+// and the new-direction sampling inside the step, so a wave pays for them
+// with only the lanes that need them active (Fresnel: readout and polished
+// hits; rotation: Rayleigh scatters and Lambertian reflections, with the
+// Cardano cosine for the scatters only). Here each step is split at those
+// branches, the per-interaction design of the reference's next steps
+// (README.md:717-720) kept inside one launch:
+//   1. every lane runs the common part (photon.hpp advance()) on its own
+//      photon in registers and, if it needs a branch body, appends a work
+//      item to a block-wide LDS queue: Rayleigh items from the front of the
+//      rotation queue, Lambertian items from its back, Fresnel items to
+//      their own queue;
+//   2. barrier; the block's threads process the queues lane-dense, Fresnel
+//      items first, then Rayleigh, then Lambertian, each type starting on a
+//      wave boundary, and write each result (new direction, or the Fresnel
+//      decision) into the owner's exchange slot;
+//   3. barrier; every owner takes its result and completes the step.
+// Queue counters are kept per step parity, so each step's counters are
+// cleared in the other step's phase 2 and no third barrier is needed. The
+// loop is block-uniform: every thread reaches both barriers every step, and
+// the block leaves once no lane holds a photon and no wave can claim one.
+// Per photon the words drawn and the arithmetic are those of the fused step.
+template <bool kCount, int kSurf>
+__global__ __launch_bounds__(1024) void wavefront_phase_kernel(DevParams P0, PhiloxKey key,
+                                                               uint64_t first_id, uint64_t n,
+                                                               uint32_t steps_per_refill,
+                                                               uint32_t chunk,
+                                                               unsigned long long* queue,
+                                                               LaunchOut out) {
+  const DevParams& P = P0;
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t ring_off = (out.tally.words + 3u) & ~3u;
+  uint32_t* cnt = lds + ring_off + RingStream::kSlots * B;  // [2 parities][4]
+  uint32_t* rot_k = cnt + 8;  // owner | code << 16
+  uint32_t* rot_c = rot_k + B;
+  uint32_t* rot_p = rot_c + B;
+  uint32_t* fr_k = rot_p + B;  // owner | face << 16
+  uint32_t* fr_w = fr_k + B;
+  uint32_t* fr_c = fr_w + B;   // cos_i bits
+  float* ex = reinterpret_cast<float*>(fr_c + B);  // exchange slot of each owner
+  float* ey = ex + B;
+  float* ez = ey + B;
+  zero_lds(lds, out.tally.words);
+  if (tid < 8) cnt[tid] = 0u;
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  WaveCounters<kCount> wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;
+
+  uint64_t q_next = 0, q_end = 0;
+  bool q_done = false;
+  bool alive = false, fresh = false, pending = false;
+  uint32_t done = 0, fate = kMaxSteps;
+  RingStream rng;
+  rng.ring = (lds_u32*)(lds + ring_off) + tid;
+  rng.stride = B;
+  rng.pos = 0u;
+  rng.wr = 0u;
+  rng.id_lo = rng.id_hi = 0u;
+  Photon ph;
+  const uint32_t emit_words = P.source == kVolume ? 6u : 3u;
+  const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
+  constexpr uint32_t kScatterCode = 6u;
+
+  for (uint32_t it = 0;; ++it) {
+    // ---- refill point (per wave, as in the ring kernel) ----------------------
+    const bool all_dead = __ballot(alive) == 0;
+    if (all_dead || it % steps_per_refill == 0) {
+      if (pending) {
+        finish<false, true>(P, out, lds, 0, ph, fate, snb, sns);
+        pending = false;
+      }
+      const uint64_t dead = __ballot(!alive);
+      if (dead != 0 && !q_done) {
+        const uint32_t need = popc(dead);
+        const uint64_t have = q_end - q_next;
+        const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+        const uint64_t b1 = q_next;
+        q_next += t1;
+        uint64_t b2 = 0;
+        uint32_t t2 = 0;
+        if (t1 < need) {
+          unsigned long long got = 0;
+          if (lane == 0) got = atomicAdd(queue, static_cast<unsigned long long>(chunk));
+          got = first_lane_u64(got);
+          if (got >= n) {
+            q_done = true;
+          } else {
+            q_end = (got + chunk < n) ? got + chunk : n;
+            const uint64_t avail = q_end - got;
+            t2 = (need - t1 < avail) ? need - t1 : static_cast<uint32_t>(avail);
+            b2 = got;
+            q_next = got + t2;
+          }
+        }
+        const uint32_t r = rank_in(dead);
+        if (!alive && r < t1 + t2) {
+          rng.reset(first_id + (r < t1 ? b1 + r : b2 + (r - t1)));
+          fresh = true;
+        }
+        wc.add(kCntEmitted, t1 + t2);
+      }
+    }
+
+    // ---- generation passes and emission (per wave) ----------------------------
+    const bool drawing = alive || fresh;
+    const uint32_t need = fresh ? emit_words + 5u : 5u;
+    while (__ballot(drawing && rng.level() < need) != 0) {
+      if (drawing && rng.room()) rng.generate(key);
+    }
+    if (fresh) {
+      emit(P, rng, ph);
+      fresh = false;
+      done = 0;
+      fate = kMaxSteps;
+      alive = P.max_steps > 0u;
+      if (!alive) pending = true;  // max_steps == 0: finished at emission
+    }
+
+    // ---- phase 1: the common part; branch work onto the queues ---------------
+    const uint64_t act = __ballot(alive);
+    StepInfo info{};
+    Advance a{};
+    int kind = -1;  // 0 Rayleigh, 1 Lambertian, 2 Fresnel
+    bool finished = false;
+    uint32_t wc_word = 0, wp_word = 0;
+    if (alive) {
+      a = advance(P, rng, ph);
+      const bool readout = a.boundary && a.face == kReadoutFace;
+      const bool side = a.boundary && a.face != kReadoutFace;
+      info.absorb = a.absorb;
+      info.scatter = a.scatter;
+      info.readout = readout;
+      info.polished = side && surface == kPolished;
+      info.black = side && surface == kBlack;
+      info.refl = side && surface >= kSpecular;
+      uint32_t used = 2u;
+      if (a.absorb) {
+        finished = true;
+        fate = kAbsorbedBulk;
+      } else if (a.scatter) {
+        kind = 0;
+        wc_word = rng.template peek<2>();
+        wp_word = rng.template peek<3>();
+        used = 4u;
+      } else if (readout || info.polished) {
+        kind = 2;
+        wc_word = rng.template peek<2>();
+        used = 3u;
+      } else if (info.black) {
+        finished = true;
+        fate = kAbsorbedSurface;
+      } else {  // reflector: u <= R survives (transport.hpp:134-137)
+        used = 3u;
+        info.survived = to_unit(rng.template peek<2>()) <= P.reflectivity;
+        if (!info.survived) {
+          finished = true;
+          fate = kAbsorbedSurface;
+        } else if (surface == kLambertian) {
+          kind = 1;
+          info.lambert_reflect = true;
+          wc_word = rng.template peek<3>();
+          wp_word = rng.template peek<4>();
+          used = 5u;
+        } else {
+          flip_axis(ph.d, a.axis);
+        }
+      }
+      rng.pos += used;
+    }
+    uint32_t* c = cnt + 4u * (it & 1u);
+    const uint32_t p_s = reserve(&c[0], kind == 0);
+    const uint32_t p_l = reserve(&c[1], kind == 1);
+    const uint32_t p_f = reserve(&c[2], kind == 2);
+    if (kind == 0) {
+      rot_k[p_s] = tid | (kScatterCode << 16);
+      rot_c[p_s] = wc_word;
+      rot_p[p_s] = wp_word;
+      ex[tid] = ph.d.x;
+      ey[tid] = ph.d.y;
+      ez[tid] = ph.d.z;
+    } else if (kind == 1) {
+      const uint32_t j = B - 1u - p_l;
+      rot_k[j] = tid | (static_cast<uint32_t>(a.face) << 16);
+      rot_c[j] = wc_word;
+      rot_p[j] = wp_word;
+    } else if (kind == 2) {
+      fr_k[p_f] = tid | (static_cast<uint32_t>(a.face) << 16);
+      fr_w[p_f] = wc_word;
+      fr_c[p_f] = __float_as_uint(a.cos_i);
+    }
+    // barrier 1, and the block's exit test: no photon in any lane and no wave
+    // that can still claim one
+    if (__syncthreads_or(alive || !q_done) == 0) break;
+
+    // ---- phase 2: the queues, lane-dense --------------------------------------
+    const uint32_t n_s = c[0], n_l = c[1], n_f = c[2];
+    if (tid < 4) cnt[4u * ((it + 1u) & 1u) + tid] = 0u;  // the next step's counters
+    const uint32_t f_pad = (n_f + 63u) & ~63u;
+    const uint32_t s_pad = (n_s + 63u) & ~63u;
+    const uint32_t total = f_pad + s_pad + n_l;
+    for (uint32_t v = tid; v < total; v += B) {
+      if (v < f_pad) {
+        if (v < n_f) {
+          const uint32_t k = fr_k[v];
+          const uint32_t owner = k & 0xffffu;
+          const bool readout = (k >> 16) == static_cast<uint32_t>(kReadoutFace);
+          bool scored;
+          const float prob = fresnel_prob(P, readout, __uint_as_float(fr_c[v]), scored);
+          ex[owner] = to_unit(fr_w[v]) <= prob ? 1.0f : 0.0f;
+        }
+      } else if (v < f_pad + s_pad) {
+        const uint32_t j = v - f_pad;
+        if (j < n_s) {
+          const uint32_t owner = rot_k[j] & 0xffffu;
+          const V3 d = redirect(true, V3{ex[owner], ey[owner], ez[owner]}, 0, false, rot_c[j],
+                                rot_p[j]);
+          ex[owner] = d.x;
+          ey[owner] = d.y;
+          ez[owner] = d.z;
+        }
+      } else {
+        const uint32_t j = B - 1u - (v - f_pad - s_pad);
+        const uint32_t k = rot_k[j];
+        const uint32_t owner = k & 0xffffu;
+        const int face = static_cast<int>(k >> 16);
+        const V3 d = redirect(false, V3{0.0f, 0.0f, 0.0f}, face >> 1, face & 1, rot_c[j],
+                              rot_p[j]);
+        ex[owner] = d.x;
+        ey[owner] = d.y;
+        ez[owner] = d.z;
+      }
+    }
+    __syncthreads();  // barrier 2
+
+    // ---- phase 3: every owner completes its step -------------------------------
+    if (kind == 0 || kind == 1) {
+      ph.d = V3{ex[tid], ey[tid], ez[tid]};
+    } else if (kind == 2) {
+      info.survived = ex[tid] != 0.0f;
+      if (info.survived) {
+        flip_axis(ph.d, a.axis);
+      } else {
+        finished = true;
+        fate = info.readout ? kDetected : kEscaped;
+      }
+    }
+    if (alive) {
+      ++done;
+      if (!finished && done >= P.max_steps) {
+        finished = true;
+        fate = kMaxSteps;
+      }
+      if (finished) {
+        alive = false;
+        pending = true;
+      }
+    }
+    wc.add_step(act, info);
+    if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+  }
+  __syncthreads();
+  flush_block<true, kCount>(out, lds, snb, sns, wc);
+}
+
+// ---------------------------------------------------------------------------
 // Batched events with the photons' random streams in LDS rings (see
 // wavefront_ring_kernel): the slot queue and per-event parameters of
 // batch_kernel, the ring streams, generation passes and deferred tallying of
@@ -1211,6 +1481,15 @@ void launch_wavefront_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_i
 }
 
 template <bool C, int S>
+void launch_wavefront_phase_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
+                              const WavefrontShape& s, unsigned long long* queue,
+                              const LaunchOut& out, hipStream_t stream) {
+  hipLaunchKernelGGL((wavefront_phase_kernel<C, S>), dim3(s.grid), dim3(s.block),
+                     phase_lds_bytes(out.tally.words, s.block), stream, P, key, first_id, n,
+                     s.steps_per_refill, s.queue_chunk, queue, out);
+}
+
+template <bool C, int S>
 void launch_wavefront_next_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                              const WavefrontShape& s, unsigned long long* queue,
                              const LaunchOut& out, hipStream_t stream) {
@@ -1233,6 +1512,23 @@ template <bool C>
 void launch_wavefront_tally(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                             const WavefrontShape& s, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream) {
+  if (s.sorted_phases) {
+    switch (P.surface) {
+      case kPolished:
+        launch_wavefront_phase_t<C, kPolished>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      case kBlack:
+        launch_wavefront_phase_t<C, kBlack>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      case kSpecular:
+        launch_wavefront_phase_t<C, kSpecular>(P, key, first_id, n, s, queue, out, stream);
+        break;
+      default:
+        launch_wavefront_phase_t<C, kLambertian>(P, key, first_id, n, s, queue, out, stream);
+        break;
+    }
+    return;
+  }
   if (s.stream_in_lds) {
     switch (P.surface) {
       case kPolished:
@@ -1298,6 +1594,10 @@ int blocks_per_cu(K kernel, int block, size_t lds) {
 size_t ring_lds_bytes(uint32_t tally_words, int block) {
   return (((static_cast<size_t>(tally_words) + 3u) & ~size_t{3}) +
           static_cast<size_t>(RingStream::kSlots) * block) * sizeof(uint32_t);
+}
+
+size_t phase_lds_bytes(uint32_t tally_words, int block) {
+  return ring_lds_bytes(tally_words, block) + (8u + 9u * static_cast<size_t>(block)) * 4u;
 }
 
 // Instantiated variants: records (no counters), tally with / without
@@ -1438,8 +1738,11 @@ int history_blocks_per_cu(bool tally, int block, size_t lds) {
 }
 
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds, bool successor,
-                            bool ring) {
+                            bool ring, bool phases) {
   if (records) return blocks_per_cu(wavefront_kernel<true, false, true>, block, lds);
+  if (phases)
+    return blocks_per_cu(wavefront_phase_kernel<false, kLambertian>, block,
+                         phase_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
   if (ring)
     return blocks_per_cu(wavefront_ring_kernel<false, kLambertian>, block,
                          ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));

@@ -30,4 +30,16 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 
 __device__ __forceinline__ uint32_t popc(uint64_t m) { return __popcll(m); }
 
+// Entry of a block-local queue for the lanes with `want`: one LDS atomic on
+// `counter` per wave. Called by all lanes of the wave.
+__device__ __forceinline__ uint32_t reserve(uint32_t* counter, bool want) {
+  const uint64_t m = __ballot(want);
+  uint32_t base = 0;
+  if (m != 0) {
+    if (lane_id() == 0) base = atomicAdd(counter, popc(m));
+    base = __builtin_amdgcn_readfirstlane(base);
+  }
+  return base + rank_in(m);
+}
+
 }  // namespace ophip

@@ -45,6 +45,7 @@ import math
 FP32_VECTOR_PEAK_TFLOPS = 157.3
 VALU_ISSUE_PEAK_TOPS = 256 * 128 * 2.4e9 / 1e12  # 78.6 lane-ops per second (x 1e12)
 HBM_PEAK_GBS = 8000.0
+HBM_PEAK_BYTES_PER_S = HBM_PEAK_GBS * 1e9
 
 DRAW = 98.0 / 4.0 + 4.0
 LOG = 29.0
@@ -97,3 +98,31 @@ def algorithmic_ops(params, counters: dict, fate_counts) -> float:
 
 def photon_steps(counters: dict) -> int:
     return int(counters["useful_lane_steps"])
+
+
+# Device-memory bytes of the per-interaction path (photon state in HBM as 12
+# 4-byte arrays per slot), per event of the workload:
+SORTED_SCAN = 4            # classify reads every slot's photon index each iteration
+SORTED_STATE_IN = 44       # a step of a photon already in flight: p, d, t, path, nb, ns, draws
+SORTED_STATE_OUT = 28      # after a step: p, t, path, draws and nb or ns
+SORTED_EMIT_OUT = 20       # a new photon also writes idx, d and both counts
+SORTED_ROT_ITEM = 12 + 12 + 12  # queue entry written and read, new direction written
+SORTED_RAYLEIGH_DIR = 12   # the rotation kernel reads a scattered photon's direction
+SORTED_FRES_ITEM = 8 + 8 + 4 + 8 + 4  # entry written and read, d component, counts, write-back
+SORTED_DETECTED = 12       # t, x, y of a detected photon for the histograms
+
+
+def sorted_path_bytes(counters: dict, n: int, slots: int, iterations: int,
+                      detected: int) -> float:
+    """Algorithmic device-memory bytes of one run of the per-interaction path,
+    from the branch-event counts of the same photons (every step counted as
+    if it ran in the sorted kernels; the history-style tail takes < 1 % of the
+    steps)."""
+    c = counters
+    steps = c["useful_lane_steps"]
+    rot = c["scatter"] + c["lambert_reflect"]
+    fres = c["readout"] + c["polished"]
+    return float(iterations * slots * SORTED_SCAN + (steps - n) * SORTED_STATE_IN
+                 + steps * SORTED_STATE_OUT + n * SORTED_EMIT_OUT + rot * SORTED_ROT_ITEM
+                 + c["scatter"] * SORTED_RAYLEIGH_DIR + fres * SORTED_FRES_ITEM
+                 + detected * SORTED_DETECTED)

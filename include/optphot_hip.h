@@ -121,6 +121,12 @@ typedef struct optphot_wavefront_tuning {
    * ring (generation passes shared across steps; takes precedence over
    * successor_threshold). Results are identical either way. */
   uint32_t stream_in_lds;
+  /* Wavefront tally only: nonzero selects the LDS-ring kernel whose steps
+   * are split at their branches: the Fresnel decisions and the new
+   * directions of a step are sorted onto block-wide LDS queues by
+   * interaction type and computed lane-dense between two block barriers
+   * (takes precedence over the two fields above). Results are identical. */
+  uint32_t sorted_phases;
 } optphot_wavefront_tuning;
 
 /* ---- Reference GPU backend entry points ----------------------------------- */

@@ -129,7 +129,11 @@ def test_tally_equals_oracle(backend, oracle_portable, name):
                    dict(block_size=64, queue_chunk=64, successor_threshold=1),
                    # Philox windows in an LDS ring, with deferred tallying
                    dict(block_size=512, steps_per_refill=2, queue_chunk=512, stream_in_lds=1),
-                   dict(block_size=64, steps_per_refill=5, queue_chunk=64, stream_in_lds=1)):
+                   dict(block_size=64, steps_per_refill=5, queue_chunk=64, stream_in_lds=1),
+                   # each step's Fresnel and rotation work sorted onto LDS queues
+                   dict(block_size=512, steps_per_refill=2, queue_chunk=512, sorted_phases=1),
+                   dict(block_size=64, steps_per_refill=1, queue_chunk=64, sorted_phases=1),
+                   dict(block_size=256, steps_per_refill=3, queue_chunk=256, sorted_phases=1)):
         t, tm, st, cnt = backend.run_gpu_tally_wavefront(p, c, n, tuning=tuning)
         np.testing.assert_array_equal(t.flat(), backend.Tally.from_flat(
             np.concatenate([ref["fate_counts"], ref["time_hist"], ref["xy_hist"].ravel(),

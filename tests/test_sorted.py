@@ -96,3 +96,22 @@ def test_sorted_equals_fused_at_scale(backend, name):
     t, _, st = backend.run_gpu_tally_sorted(p, c, n)
     np.testing.assert_array_equal(t.flat(), fused.flat())
     assert st.emitted == n and st.iterations >= 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["c2_long", "wrapped_lambertian", "every_process", "mirror_trap"])
+def test_phase_kernel_equals_ring_kernel_at_scale(backend, name):
+    """The fused kernel with LDS-sorted branch phases against the plain ring
+    kernel at 2e7 photons, and at a block size whose waves hold few items."""
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = configs.PARITY[name]
+    c = TallyConfig(256, 100.0, 64)
+    n = 20_000_000
+    ring, _, _, _ = backend.run_gpu_tally_wavefront(
+        p, c, n, counters=False,
+        tuning=dict(block_size=512, steps_per_refill=2, queue_chunk=512, stream_in_lds=1))
+    for tuning in (dict(block_size=512, steps_per_refill=2, queue_chunk=512, sorted_phases=1),
+                   dict(block_size=128, steps_per_refill=4, queue_chunk=128, sorted_phases=1)):
+        t, _, st, cnt = backend.run_gpu_tally_wavefront(p, c, n, tuning=tuning)
+        np.testing.assert_array_equal(t.flat(), ring.flat(), err_msg=str(tuning))
+        assert cnt.as_dict()["emitted"] == n
