@@ -167,7 +167,7 @@ struct Context {
   std::vector<hipEvent_t> piece_ev;  // records runs: launch start / end, copy done
   std::unique_ptr<CopyPool> pool;
   DevBuf partials, scratch, tally;
-  DevBuf batch_table, batch_offsets, batch_counts;
+  DevBuf batch_table, batch_offsets, batch_counts, batch_order;
   DevBuf adj_weights, adj_grad, adj_queue;
   // Captured graph of the batched launch and the arguments it was built for.
   hipGraphExec_t batch_exec = nullptr;
@@ -1094,11 +1094,21 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     const uint32_t words = 6u + time_bins;
     const size_t count_words = static_cast<size_t>(n_events) * words;
     const size_t zero_u64 = (count_words + 1) / 2;
+    // Queue order: events grouped by branch structure (stable within a group).
+    std::vector<uint32_t> order(n_events);
+    for (uint32_t e = 0; e < n_events; ++e) order[e] = e;
+    auto group = [&](uint32_t e) {
+      return params[e].surface * 4u + (params[e].scat_length != kInf ? 2u : 0u) +
+             (params[e].abs_length != kInf ? 1u : 0u);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return group(a) < group(b); });
 
     const auto ta = std::chrono::steady_clock::now();
     bool grew = c.batch_table.ensure(table.size() * sizeof(DevParams));
     grew |= c.batch_offsets.ensure(offsets.size() * sizeof(unsigned long long));
     grew |= c.batch_counts.ensure(zero_u64 * 8);
+    grew |= c.batch_order.ensure(order.size() * sizeof(uint32_t));
     if (grew) {
       tm.alloc_ms += ms_since(ta);
       c.drop_graphs();  // buffers moved: captured graphs are stale
@@ -1131,6 +1141,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     HIPCHK(hipMemcpyAsync(c.batch_offsets.ptr, offsets.data(),
                           offsets.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
                           c.stream));
+    HIPCHK(hipMemcpyAsync(c.batch_order.ptr, order.data(), order.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     tm.h2d_ms = ms_since(th);
 
@@ -1143,6 +1155,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     lo.evt.time_scale = static_cast<float>(time_bins) / time_max;  // transport.hpp:171
     lo.evt.table = c.batch_table.as<DevParams>();
     lo.evt.offsets = c.batch_offsets.as<unsigned long long>();
+    lo.evt.order = c.batch_order.as<uint32_t>();
     lo.evt.n_events = n_events;
     const PhiloxKey key = key_of(params[0]);
 

@@ -53,6 +53,8 @@ struct DevEventTally {
   float time_scale;                 // time_bins / time_max
   const DevParams* table;           // [n_events] per-event parameters (batched kernel)
   const unsigned long long* offsets;  // [n_events + 1] prefix sums of photons per event
+  const uint32_t* order;            // [n_events] queue order: slot c is chunk c / E of
+                                    // event order[c mod E] (nullptr: event c mod E)
   uint32_t n_events;
   uint32_t event;                   // the single event of a plain per-event launch
 };
@@ -190,7 +192,7 @@ void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
 // first_id + offsets[e] .. first_id + offsets[e+1] - 1), per-lane event
 // parameters from out.evt.table, per-event tallies. The queue has n_slots =
 // n_events * max_e ceil(photons_e / queue_chunk) slots; slot c is chunk
-// c / n_events of event c mod n_events (empty slots are skipped).
+// c / n_events of event order[c mod n_events] (empty slots are skipped).
 void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
                             const WavefrontShape& shape, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream);

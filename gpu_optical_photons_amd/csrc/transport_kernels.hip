@@ -833,6 +833,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   const uint32_t lane = lane_id();
   const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
+  const uint32_t* ord = out.evt.order;
   WaveCounters<kCount> wc;
   wc.clear();
   uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
@@ -883,7 +884,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
             s_end = (c + per_claim < n_slots) ? c + per_claim : n_slots;
           }
           const uint64_t c = s_next++;
-          const uint32_t e = static_cast<uint32_t>(c % E);
+          const uint32_t e = ord ? ord[c % E] : static_cast<uint32_t>(c % E);
           const uint64_t start = (c / E) * chunk;
           const uint64_t ne = off[e + 1] - off[e];
           if (start >= ne) continue;
@@ -1065,9 +1066,13 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
 // ---------------------------------------------------------------------------
 // Batched events. Lanes take photons of many events and carry their event's
 // parameters in registers (loaded when the lane is refilled). The queue hands
-// out slots c = 0, 1, ...; slot c is chunk c / E of event c mod E, so the
-// waves in flight at any moment work on many different events and the
+// out slots c = 0, 1, ...; slot c is chunk c / E of event order[c mod E], so
+// the waves in flight at any moment work on many different events and the
 // per-event tally atomics spread over many rows instead of queueing on one.
+// The host orders the events by their branch structure (surface model,
+// scattering on, absorption on), so the consecutive slots a wave claims
+// mostly share it: branch bodies that none of the wave's events need (the
+// rotation when no event scatters or reflects diffusely) are then skipped.
 //
 // With kLds (few events: n_events * words u32 fit in LDS) each block keeps
 // its own copy of all event tallies in LDS and adds it to the global rows
@@ -1089,6 +1094,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
   const uint32_t lane = lane_id();
   const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
+  const uint32_t* ord = out.evt.order;
   WaveCounters<kCount> wc;
   wc.clear();
   uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
@@ -1129,7 +1135,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
           s_end = (c + per_claim < n_slots) ? c + per_claim : n_slots;
         }
         const uint64_t c = s_next++;
-        const uint32_t e = static_cast<uint32_t>(c % E);
+        const uint32_t e = ord ? ord[c % E] : static_cast<uint32_t>(c % E);
         const uint64_t start = (c / E) * chunk;
         const uint64_t ne = off[e + 1] - off[e];
         if (start >= ne) continue;
