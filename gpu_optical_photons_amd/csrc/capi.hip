@@ -168,6 +168,7 @@ struct Context {
   std::unique_ptr<CopyPool> pool;
   DevBuf partials, scratch, tally;
   DevBuf batch_table, batch_offsets, batch_counts, batch_order;
+  DevBuf rayleigh;  // 2^24 Rayleigh cosines (DevParams::rayleigh), built at init
   DevBuf adj_weights, adj_grad, adj_queue;
   // Captured graph of the batched launch and the arguments it was built for.
   hipGraphExec_t batch_exec = nullptr;
@@ -212,6 +213,10 @@ struct Context {
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     pool = std::make_unique<CopyPool>(std::min(8u, std::max(1u, hw / 2)) - 1);
+    rayleigh.ensure(kRayleighTableEntries * sizeof(float));
+    launch_rayleigh_table(rayleigh.as<float>(), stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
     ready = true;
   }
 
@@ -265,6 +270,7 @@ DevParams to_dev(const optphot_sim_params& p) {
   d.surface = p.surface;
   d.source = p.source;
   d.max_steps = p.max_steps;
+  d.rayleigh = ctx().rayleigh.as<const float>();
   return d;
 }
 
@@ -698,6 +704,17 @@ __attribute__((visibility("default"))) int optphot_gpu_device_name(char* buf, si
     std::lock_guard<std::mutex> lock(c.mu);
     c.init();
     std::snprintf(buf, len, "%s", c.name.c_str());
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_rayleigh_table(float* out, size_t n) {
+  return guarded([&] {
+    if (!out || n != kRayleighTableEntries)
+      throw InvalidArgument("rayleigh table: out must hold 2^24 floats");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    HIPCHK(hipMemcpy(out, c.rayleigh.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
   });
 }
 

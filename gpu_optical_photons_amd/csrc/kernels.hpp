@@ -230,6 +230,10 @@ void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials
 // Fills n u64 words with zero (a kernel, so it may sit inside captured graphs).
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
 
+// Fills the 2^24-entry Rayleigh-cosine table (DevParams::rayleigh).
+constexpr size_t kRayleighTableEntries = size_t{1} << 24;
+void launch_rayleigh_table(float* table, hipStream_t stream);
+
 // Persistent transport of genstep photons into the block tallies (and
 // records at each photon's global index when out.records).
 void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,

@@ -55,6 +55,10 @@ struct DevParams {
   float tau;
   uint32_t det_matched, out_matched;  // n_scint == n_det, n_scint == n_out
   uint32_t surface, source, max_steps;
+  // Rayleigh cosines of every 24-bit uniform (2^24 entries, entry k for
+  // u = (k + 1) 2^-24, filled on the device by rayleigh_cos); nullptr: the
+  // step computes them.
+  const float* rayleigh;
 };
 
 // ---------------------------------------------------------------------------
@@ -323,6 +327,15 @@ __device__ __forceinline__ float rayleigh_cos(float u) {
   const float c = p_cbrt(pos ? q + s : s - q);
   const float a = pos ? c : div_n(1.0f, c);  // c in [1, 1.62], so a in [0.61, 1.62]
   return __builtin_fminf(1.0f, __builtin_fmaxf(-1.0f, a - div_n(1.0f, a)));
+}
+
+// Rayleigh cosine of the uniform drawn from word w (to_unit keeps its top 24
+// bits), from the table when the run has one. The table entries are this
+// function's own values, so both give the same bits; the table replaces ~90
+// instructions that a wave runs whenever any of its lanes scatters.
+__device__ __forceinline__ float rayleigh_cos_word(const float* table, uint32_t w) {
+  if (table) return table[w >> 8];
+  return rayleigh_cos(to_unit(w));
 }
 
 // Tally binning (transport.hpp:170-181); `time_scale` = time_bins / time_max
@@ -623,14 +636,14 @@ __device__ __forceinline__ void flip_axis(V3& d, int axis) {
 // cos from Cardano, sampling.hpp:39-49) or Lambertian reflection (about the
 // inward normal -n of the face on `axis`, cos = sqrt(u), sampling.hpp:53-55),
 // from the two words wc (cos) and wp (azimuth).
-__device__ __forceinline__ V3 redirect(bool scatter, V3 d, int axis, bool plus, uint32_t wc,
-                                       uint32_t wp) {
-  const float uc = to_unit(wc), up = to_unit(wp);
+__device__ __forceinline__ V3 redirect(const DevParams& P, bool scatter, V3 d, int axis, bool plus,
+                                       uint32_t wc, uint32_t wp) {
+  const float up = to_unit(wp);
   float cos_theta;
   if (scatter)
-    cos_theta = rayleigh_cos(uc);
+    cos_theta = rayleigh_cos_word(P.rayleigh, wc);
   else
-    cos_theta = sqrt_n(uc);  // uc in [2^-24, 1]
+    cos_theta = sqrt_n(to_unit(wc));  // u in [2^-24, 1]
   const float s = plus ? 1.0f : -1.0f;
   // -n keeps the reference's signed zeros: -(+0) = -0 off-axis.
   const V3 inward{axis == 0 ? -s : -0.0f, axis == 1 ? -s : -0.0f, axis == 2 ? -s : -0.0f};
@@ -684,7 +697,7 @@ __device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, u
     // Rayleigh: draws 2, 3. Lambertian: draws 3, 4.
     const uint32_t wc = scatter ? rng.template peek<2>() : rng.template peek<3>();
     const uint32_t wp = scatter ? rng.template peek<3>() : rng.template peek<4>();
-    ph.d = redirect(scatter, ph.d, a.axis, a.plus, wc, wp);
+    ph.d = redirect(P, scatter, ph.d, a.axis, a.plus, wc, wp);
   }
   rng.pos += 2u + (decide ? 1u : 0u) + ((scatter || lambert_reflect) ? 2u : 0u);
 

@@ -340,7 +340,8 @@ __global__ __launch_bounds__(kSortedBlock) void sorted_fresnel_kernel(DevParams 
 
 // ---------------------------------------------------------------------------
 // rotate: new directions for Rayleigh scattering and Lambertian reflection.
-__global__ __launch_bounds__(kSortedBlock) void sorted_rotate_kernel(SortedArgs A) {
+__global__ __launch_bounds__(kSortedBlock) void sorted_rotate_kernel(DevParams P0, SortedArgs A) {
+  const DevParams& P = P0;
   const uint32_t b = blockIdx.x;
   const uint32_t base = b * A.seg;
   const uint32_t count = A.cnt_rot[b];
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(kSortedBlock) void sorted_rotate_kernel(SortedArgs 
     const int face = scatter ? 0 : static_cast<int>(code);
     V3 d{0.0f, 0.0f, 0.0f};
     if (scatter) d = V3{A.dx[s], A.dy[s], A.dz[s]};
-    d = redirect(scatter, d, face >> 1, face & 1, wc, wp);
+    d = redirect(P, scatter, d, face >> 1, face & 1, wc, wp);
     A.dx[s] = d.x;
     A.dy[s] = d.y;
     A.dz[s] = d.z;
@@ -456,7 +457,7 @@ void launch_sorted_iteration(const DevParams& P, PhiloxKey key, const SortedArgs
   const size_t lds = static_cast<size_t>(A.tally.words) * sizeof(uint32_t);
   sorted_classify_kernel<<<A.grid, kSortedBlock, 0, stream>>>(P, key, A);
   sorted_fresnel_kernel<<<A.grid, kSortedBlock, lds, stream>>>(P, A);
-  sorted_rotate_kernel<<<A.grid, kSortedBlock, 0, stream>>>(A);
+  sorted_rotate_kernel<<<A.grid, kSortedBlock, 0, stream>>>(P, A);
 }
 
 void launch_sorted_status(const SortedArgs& A, hipStream_t stream) {

@@ -759,7 +759,7 @@ __global__ __launch_bounds__(1024) void wavefront_phase_kernel(DevParams P0, Phi
         const uint32_t j = v - f_pad;
         if (j < n_s) {
           const uint32_t owner = rot_k[j] & 0xffffu;
-          const V3 d = redirect(true, V3{ex[owner], ey[owner], ez[owner]}, 0, false, rot_c[j],
+          const V3 d = redirect(P, true, V3{ex[owner], ey[owner], ez[owner]}, 0, false, rot_c[j],
                                 rot_p[j]);
           ex[owner] = d.x;
           ey[owner] = d.y;
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(1024) void wavefront_phase_kernel(DevParams P0, Phi
         const uint32_t k = rot_k[j];
         const uint32_t owner = k & 0xffffu;
         const int face = static_cast<int>(k >> 16);
-        const V3 d = redirect(false, V3{0.0f, 0.0f, 0.0f}, face >> 1, face & 1, rot_c[j],
+        const V3 d = redirect(P, false, V3{0.0f, 0.0f, 0.0f}, face >> 1, face & 1, rot_c[j],
                               rot_p[j]);
         ex[owner] = d.x;
         ey[owner] = d.y;
@@ -834,6 +834,8 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
   const uint32_t* ord = out.evt.order;
+  // every event carries the same table pointer; read it once, as a uniform
+  const float* const rtab = out.evt.table[0].rayleigh;
   WaveCounters<kCount> wc;
   wc.clear();
   uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
@@ -901,6 +903,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
           pset = r < t1 ? s1 : we;
           const uint64_t idx = r < t1 ? b1 + r : b2 + (r - t1);
           P = out.evt.table[pset];
+          P.rayleigh = rtab;
           rng.reset(first_id + off[pset] + idx);
           fresh = true;
         }
@@ -1095,6 +1098,8 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
   const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
   const uint32_t* ord = out.evt.order;
+  // every event carries the same table pointer; read it once, as a uniform
+  const float* const rtab = out.evt.table[0].rayleigh;
   WaveCounters<kCount> wc;
   wc.clear();
   uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
@@ -1153,6 +1158,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
         pset = r < t1 ? s1 : we;
         idx = r < t1 ? b1 + r : b2 + (r - t1);
         P = out.evt.table[pset];
+        P.rayleigh = rtab;
         rng.start(first_id + off[pset] + idx, key, 0u);
         emit(P, rng, ph);
         done = 0;
@@ -1311,7 +1317,10 @@ __global__ __launch_bounds__(1024) void adjoint_kernel(DevParams P0, PhiloxKey k
       if (gets) {
         pset = nset;
         const uint64_t local = r < t1 ? b1 + r : b2 + (r - t1);
-        if (!kUni) Pv = a.table[pset];
+        if (!kUni) {
+          Pv = a.table[pset];
+          Pv.rayleigh = P0.rayleigh;  // the same for every set; uniform
+        }
         rng.start(first_id + a.offsets[pset] + local, key, 0u);
         emit(P, rng, ph);
         done = 0;
@@ -1387,6 +1396,14 @@ __global__ void tally_reduce_kernel(uint32_t words, int blocks, const uint32_t* 
     for (int b = blockIdx.y; b < blocks; b += gridDim.y) s += sums[2 * b + j];
   }
   if (w < words + 2u && s != 0) atomicAdd(tally + w, s);
+}
+
+// Entry k = rayleigh_cos of u = (k + 1) 2^-24, the uniform of every word whose
+// top 24 bits are k (philox.hpp:82-84).
+__global__ void rayleigh_table_kernel(float* table) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < (1u << 24); k += stride)
+    table[k] = rayleigh_cos(to_unit(k << 8));
 }
 
 __global__ void zero_u64_kernel(unsigned long long* p, uint64_t n) {
@@ -1728,6 +1745,10 @@ void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
                        hipStream_t stream) {
   hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, stream, out, iters,
                      half ? 1u : 0u);
+}
+
+void launch_rayleigh_table(float* table, hipStream_t stream) {
+  hipLaunchKernelGGL(rayleigh_table_kernel, dim3(4096), dim3(256), 0, stream, table);
 }
 
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {

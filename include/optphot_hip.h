@@ -329,6 +329,13 @@ int optphot_run_gpu_chi2_step(const optphot_sim_params* params, const uint64_t* 
  * iters x 8 VALU FMAs plus a handful of setup instructions. */
 int optphot_valu_probe(uint32_t blocks, uint32_t iters, int half_lanes, double* kernel_ms);
 
+/* Copies the device's Rayleigh-cosine table into out (n must be 2^24):
+ * entry k is the Rayleigh scattering cosine (sampling.hpp:39-44) of the
+ * uniform (k + 1) 2^-24, i.e. of every Philox word whose top 24 bits are k.
+ * The transport kernels read scattering cosines from this table
+ * (diagnostics: tests compare it with the CPU function). */
+int optphot_rayleigh_table(float* out, size_t n);
+
 /* Number of compute units and the kernels' register / occupancy facts,
  * written as "key=value" lines into buf (diagnostics). */
 int optphot_device_info(char* buf, size_t len);

@@ -687,6 +687,12 @@ ORACLE_EXPORT float oracle_box_exit(const float p[3], const float d[3], float hx
   return box_exit(v3_make(p[0], p[1], p[2]), v3_make(d[0], d[1], d[2]), hx, hy, hz, face);
 }
 ORACLE_EXPORT float oracle_rayleigh_cos(float u) { return sample_rayleigh_cos(u); }
+/* sample_rayleigh_cos of every 24-bit uniform: out[k] for u = (k + 1) 2^-24
+ * (philox.hpp:82-84, sampling.hpp:39-44); out holds 2^24 floats. */
+ORACLE_EXPORT void oracle_rayleigh_table(float* out) {
+  for (uint32_t k = 0; k < (1u << 24); ++k)
+    out[k] = sample_rayleigh_cos(u32_to_uniform(k << 8));
+}
 ORACLE_EXPORT void oracle_rotate_about(const float axis[3], float cos_theta, float phi_turns,
                                        float out[3]) {
   const v3 r = rotate_about(v3_make(axis[0], axis[1], axis[2]), cos_theta, phi_turns);

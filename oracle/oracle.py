@@ -128,6 +128,7 @@ class Oracle(_Lib):
         self._fn("run_tally", None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
                                      ctypes.c_uint64, ctypes.c_int, _u64p])
         self._fn("u32_to_uniform", ctypes.c_float, [ctypes.c_uint32])
+        self._fn("rayleigh_table", None, [_f32p])
         self._fn("adjoint", None, [ctypes.c_void_p, _u64p, ctypes.c_uint32, ctypes.c_uint64,
                                    ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_int])
@@ -190,6 +191,13 @@ class Oracle(_Lib):
         if with_draws:
             r["draws"] = draws
         return r
+
+    def rayleigh_cos_table(self):
+        """sample_rayleigh_cos of every 24-bit uniform (2^24 floats): entry k
+        for u = (k + 1) 2^-24."""
+        out = np.empty(1 << 24, np.float32)
+        self.rayleigh_table(out)
+        return out
 
     def tally(self, params, n, time_bins=100, time_max=50.0, xy_bins=50, first_id=0, threads=0):
         cfg = TallyConfigC(time_bins, time_max, xy_bins)

@@ -161,3 +161,17 @@ def test_records_against_default_build(backend, name, n):
     assert same.mean() >= 0.9999, same.mean()
     dt = np.abs(gpu.t[same] - cpu["t"][same]) / np.maximum(np.abs(cpu["t"][same]), 1e-30)
     assert (dt <= 1e-5).mean() >= 0.999
+
+
+def test_rayleigh_table_equals_oracle(backend, oracle_portable):
+    """The device's Rayleigh-cosine table (read by every transport kernel for
+    its scattering cosines) against the CPU function, all 2^24 entries, bit
+    for bit."""
+    import ctypes
+    fn = backend.lib().optphot_rayleigh_table
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_size_t]
+    dev = np.empty(1 << 24, np.float32)
+    backend._check(fn(dev.ctypes.data, dev.size))
+    ref = oracle_portable.rayleigh_cos_table()
+    np.testing.assert_array_equal(dev.view(np.uint32), ref.view(np.uint32))

@@ -425,3 +425,22 @@ def test_thread_count_first_id_and_tally(oracle):
     assert np.array_equal(t["fate_counts"], _fates(a))
     assert int(t["time_hist"].sum()) == int(_fates(a)[0])
     assert t["boundary_sum"] == int(a["n_boundary"].sum())
+
+
+def test_rayleigh_table_is_the_sampling_function(oracle_portable):
+    """The oracle's 2^24-entry table (the checker of the device table) holds
+    sample_rayleigh_cos of u = (k + 1) 2^-24, and is the inverse of the
+    Rayleigh CDF F(mu) = (mu^3 + 3 mu + 4) / 8 (sampling.hpp:30-44)."""
+    t = oracle_portable.rayleigh_cos_table()
+    rng = np.random.default_rng(5)
+    ks = np.concatenate([[0, 1, (1 << 23) - 1, 1 << 23, (1 << 24) - 1],
+                         rng.integers(0, 1 << 24, 2000)])
+    for k in ks:
+        u = oracle_portable.u32_to_uniform(int(k) << 8)
+        assert t[k] == oracle_portable.rayleigh_cos(u)
+    u = (np.arange(1 << 24, dtype=np.float64) + 1.0) / (1 << 24)
+    mu = t.astype(np.float64)
+    assert np.all(np.abs(mu) <= 1.0)
+    # monotone in u up to the float rounding of the Cardano evaluation
+    assert np.min(np.diff(mu)) > -1e-6
+    assert np.max(np.abs((mu ** 3 + 3 * mu + 4) / 8 - u)) < 4e-7
