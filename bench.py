@@ -159,6 +159,39 @@ def bench_sorted(p, c, n, fused_tally, counters, ring_tuning):
     return out
 
 
+def bench_reference_workloads():
+    """The reference's own published workloads (README.md:540-552), 1e8
+    photons in tally mode with its default binning (100 bins x 50 ns, 50^2):
+    configs/default.cfg (50 mm cube, polished faces, point source) and
+    configs/wrapped_lambertian.cfg (Lambertian R 0.95, n_det 1.5, abs 1000 mm,
+    scat 2000 mm, volume source), seed 12345. The reference's figures are for
+    its history kernel on a GTX 1660 Ti: context, not a like-for-like
+    baseline."""
+    import math
+
+    from gpu_optical_photons_amd import autotune, backend
+    from gpu_optical_photons_amd.params import SimParams, Source, Surface, TallyConfig
+    c = TallyConfig()
+    n = 100_000_000
+    cases = {
+        "default_cube": (SimParams(), 9.3e8),
+        "wrapped_lambertian": (SimParams(surface=Surface.LAMBERTIAN, reflectivity=0.95, n_det=1.5,
+                                         abs_length=1000.0, scat_length=2000.0,
+                                         source=Source.VOLUME, tau=4.0), 3.3e8),
+    }
+    out = {}
+    for name, (p, published) in cases.items():
+        assert not math.isinf(p.abs_length)
+        tu = autotune.choice_for("wavefront_tally", p, c)
+        backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)
+        ms = min(backend.run_gpu_tally_wavefront(p, c, n, tuning=tu, counters=False)[1].kernel_ms
+                 for _ in range(3))
+        out[name] = {"photons": n, "kernel_ms": ms, "photons_per_s": n / (ms * 1e-3),
+                     "reference_published_photons_per_s": published,
+                     "reference_hardware": "GTX 1660 Ti, history kernel"}
+    return out
+
+
 def bench_batched():
     """Batched events: 4096 events (box ~U[10,100] mm per axis, surface model
     uniform over polished / black / specular / Lambertian, n_det ~U[1.4,1.6],
@@ -407,6 +440,7 @@ def main():
         line["differentiation"] = bench_differentiation()
         line["calibration"] = bench_calibration()
         line["sorted_by_interaction"] = bench_sorted(p, c, n, tally, counters, tuning)
+        line["reference_workloads"] = bench_reference_workloads()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, c)
     print(json.dumps(line), flush=True)
