@@ -341,7 +341,10 @@ __global__ __launch_bounds__(kSortedBlock) void sorted_fresnel_kernel(DevParams 
 // ---------------------------------------------------------------------------
 // rotate: new directions for Rayleigh scattering and Lambertian reflection.
 __global__ __launch_bounds__(kSortedBlock) void sorted_rotate_kernel(DevParams P0, SortedArgs A) {
-  const DevParams& P = P0;
+  // The queue is lane-dense, so the Rayleigh cosines are computed here: the
+  // table would only add a random gather per item to an HBM-bound kernel.
+  DevParams P = P0;
+  P.rayleigh = nullptr;
   const uint32_t b = blockIdx.x;
   const uint32_t base = b * A.seg;
   const uint32_t count = A.cnt_rot[b];
