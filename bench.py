@@ -58,9 +58,23 @@ def cpu_baseline(p, c):
     t0 = time.perf_counter()
     lib.tally(p, n, c.time_bins, c.time_max, c.xy_bins, threads=0)
     dt = time.perf_counter() - t0
+    # one thread, about 3 s of work
+    n1 = int(max(10_000, min(n, 3.0 * n / dt / max(cores, 1))))
+    t1 = time.perf_counter()
+    lib.tally(p, n1, c.time_bins, c.time_max, c.xy_bins, threads=1)
+    d1 = time.perf_counter() - t1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": n / dt, "unit": "photons/s", "cores": cores, "kind": kind,
             "sample": f"photon ids 0..{n - 1} of the same workload (tally mode), "
-                      f"{dt:.1f} s wall on {cores} OpenMP threads"}
+                      f"{dt:.1f} s wall on {cores} OpenMP threads",
+            "cpu_model": model, "nproc": os.cpu_count(),
+            "one_thread": {"value": n1 / d1, "unit": "photons/s", "cores": 1,
+                           "sample": f"photon ids 0..{n1 - 1}, {d1:.1f} s wall"}}
 
 
 def bench_plain_small():
