@@ -127,14 +127,22 @@ def bench_gensteps():
     c = TallyConfig(256, 100.0, 64)
     g = make_gensteps(100_000, p, photons_per_mm=200.0, seed=1)
     n = int(g["n_photons"].sum())
-    tu = dict(block_size=512, steps_per_refill=2, queue_chunk=256)
-    run_gpu_gensteps(p, c, g, tuning=tu)
-    ms = []
-    for _ in range(3):
-        _, _, tm, _ = run_gpu_gensteps(p, c, g, tuning=tu)
-        ms.append(tm.kernel_ms)
-    return {"gensteps": len(g), "photons": n, "kernel_ms": min(ms),
-            "photons_per_s": n / (min(ms) * 1e-3), "upload_ms": tm.h2d_ms}
+    out = {"gensteps": len(g), "photons": n}
+    for name, tu in (("registers", dict(block_size=512, steps_per_refill=2, queue_chunk=256)),
+                     ("lds_ring", dict(block_size=512, steps_per_refill=2, queue_chunk=256,
+                                       stream_in_lds=1))):
+        run_gpu_gensteps(p, c, g, tuning=tu)
+        ms = []
+        for _ in range(3):
+            _, _, tm, _ = run_gpu_gensteps(p, c, g, tuning=tu)
+            ms.append(tm.kernel_ms)
+        out[name] = {"kernel_ms": min(ms), "photons_per_s": n / (min(ms) * 1e-3)}
+        out["upload_ms"] = tm.h2d_ms
+    best = min(("registers", "lds_ring"), key=lambda k: out[k]["kernel_ms"])
+    out["kernel"] = best
+    out["kernel_ms"] = out[best]["kernel_ms"]
+    out["photons_per_s"] = out[best]["photons_per_s"]
+    return out
 
 
 def bench_sorted(p, c, n, fused_tally, counters, ring_tuning):

@@ -1251,6 +1251,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
       if (tuning->steps_per_refill) tu.steps_per_refill = tuning->steps_per_refill;
       if (tuning->queue_chunk) tu.queue_chunk = tuning->queue_chunk;
       if (tuning->blocks_per_cu) tu.blocks_per_cu = tuning->blocks_per_cu;
+      tu.stream_in_lds = tuning->stream_in_lds;
     }
     validate_block(tu.block_size, "gensteps");
     if (tu.queue_chunk < 64 || tu.queue_chunk % 64 != 0 || tu.steps_per_refill == 0)
@@ -1278,7 +1279,13 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
     const DevTally T = to_dev_tally(*tc, *p);
     check_lds(c, T);
     const size_t lds = static_cast<size_t>(T.words) * 4;
-    const int max_per_cu = std::max(1, genstep_blocks_per_cu(records != nullptr, tu.block_size, lds));
+    // LDS rings for the Philox windows: tally runs only
+    const bool ring = tu.stream_in_lds != 0 && records == nullptr;
+    if (ring && ring_lds_bytes(T.words, tu.block_size) > c.max_lds)
+      throw InvalidArgument("gensteps: tally plus LDS streams exceed the LDS of a block; use a "
+                            "smaller block_size");
+    const int max_per_cu =
+        std::max(1, genstep_blocks_per_cu(records != nullptr, tu.block_size, lds, ring));
     const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
     WavefrontShape s{};
     s.block = tu.block_size;
@@ -1287,6 +1294,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
                               static_cast<uint64_t>(per_cu) * c.cu_count)));
     s.steps_per_refill = tu.steps_per_refill;
     s.queue_chunk = tu.queue_chunk;
+    s.stream_in_lds = ring ? 1u : 0u;
     s.slots_per_claim = slots_per_claim(slots.size(), static_cast<uint64_t>(s.grid) * s.block / 64u);
     if (total / static_cast<uint64_t>(s.grid) >= 0xffffffffull)
       throw std::runtime_error("too many photons for one tally launch; split the run");

@@ -1062,6 +1062,142 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
   flush_block<true, kCount>(out, lds, snb, sns, wc);
 }
 
+// Photons from gensteps with the Philox windows in LDS rings (tally): the
+// genstep slot queue of genstep_kernel, and the ring streams, generation
+// passes and deferred tallying of wavefront_ring_kernel. A lane keeps the
+// index of its genstep from the refill point to its emission, which draws
+// from the ring like the step.
+template <bool kCount, int kSurf>
+__global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, PhiloxKey key,
+                                                            uint64_t first_id,
+                                                            uint32_t steps_per_refill,
+                                                            uint32_t chunk, uint32_t per_claim,
+                                                            GenstepArgs g,
+                                                            unsigned long long* queue,
+                                                            LaunchOut out) {
+  const DevParams& P = P0;
+  extern __shared__ uint32_t lds[];
+  const uint32_t ring_off = (out.tally.words + 3u) & ~3u;
+  zero_lds(lds, out.tally.words);
+  __syncthreads();
+  const uint32_t lane = lane_id();
+  WaveCounters<kCount> wc;
+  wc.clear();
+  uint64_t snb = 0, sns = 0;
+
+  uint32_t wg = 0;                 // genstep of the wave's open range (uniform)
+  uint64_t q_next = 0, q_end = 0;  // open range, indices within genstep wg
+  uint64_t s_next = 0, s_end = 0;  // slots claimed by the wave, not yet opened
+  bool q_done = false;
+  bool alive = false, fresh = false, pending = false;
+  uint32_t lg = 0;  // the lane's genstep, from its refill to its emission
+  uint32_t done = 0, fate = kMaxSteps;
+  RingStream rng;
+  rng.ring = (lds_u32*)(lds + ring_off) + threadIdx.x;
+  rng.stride = blockDim.x;
+  rng.pos = 0u;
+  rng.wr = 0u;
+  rng.id_lo = rng.id_hi = 0u;
+  Photon ph;
+  constexpr uint32_t kEmitWords = 4u;  // u along the segment (if any), cos, phi, delay
+
+  for (uint32_t it = 0;; ++it) {
+    // ---- refill point: tally the finished photons, assign ids to free lanes --
+    const bool all_dead = __ballot(alive) == 0;
+    if (all_dead || it % steps_per_refill == 0) {
+      if (pending) {
+        finish<false, true>(P, out, lds, 0, ph, fate, snb, sns);
+        pending = false;
+      }
+      const uint64_t dead = __ballot(!alive);
+      if (dead != 0 && !q_done) {
+        const uint32_t need = popc(dead);
+        const uint64_t have = q_end - q_next;
+        const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
+        const uint64_t b1 = q_next;
+        const uint32_t g1 = wg;
+        q_next += t1;
+        uint64_t b2 = 0;
+        uint32_t t2 = 0;
+        if (t1 < need) {
+          if (s_next >= s_end) {
+            unsigned long long c = 0;
+            if (lane == 0) c = atomicAdd(queue, static_cast<unsigned long long>(per_claim));
+            c = first_lane_u64(c);
+            if (c >= g.n_slots) {
+              q_done = true;
+            } else {
+              s_next = c;
+              s_end = (c + per_claim < g.n_slots) ? c + per_claim : g.n_slots;
+            }
+          }
+          if (!q_done) {
+            const uint2 sl = g.slots[s_next++];
+            wg = sl.x;
+            const uint64_t ng = g.gensteps[wg].n_photons;
+            q_next = static_cast<uint64_t>(sl.y) * chunk;
+            q_end = (q_next + chunk < ng) ? q_next + chunk : ng;
+            t2 = (need - t1 < q_end - q_next) ? need - t1
+                                               : static_cast<uint32_t>(q_end - q_next);
+            b2 = q_next;
+            q_next += t2;
+          }
+        }
+        const uint32_t r = rank_in(dead);
+        if (!alive && r < t1 + t2) {
+          lg = r < t1 ? g1 : wg;
+          const uint64_t idx = g.gensteps[lg].offset + (r < t1 ? b1 + r : b2 + (r - t1));
+          rng.reset(first_id + idx);
+          fresh = true;
+        }
+        wc.add(kCntEmitted, t1 + t2);
+      }
+      if (all_dead && __ballot(fresh) == 0) {
+        if (q_done) break;
+        continue;
+      }
+    }
+
+    // ---- generation passes -------------------------------------------------
+    const bool drawing = alive || fresh;
+    const uint32_t need = fresh ? kEmitWords + 5u : 5u;
+    while (__ballot(drawing && rng.level() < need) != 0) {
+      if (drawing && rng.room()) rng.generate(key);
+    }
+
+    // ---- emission of the newly assigned photons ----------------------------
+    if (fresh) {
+      const DevGenstep G = g.gensteps[lg];
+      emit_genstep(P, G, rng, ph);
+      fresh = false;
+      done = 0;
+      fate = kMaxSteps;
+      alive = P.max_steps > 0u;
+      if (!alive) pending = true;  // max_steps == 0: finished at emission
+    }
+
+    // ---- one transport step --------------------------------------------------
+    const uint64_t act = __ballot(alive);
+    if (act == 0) continue;
+    StepInfo info{};
+    bool finished = false;
+    if (alive) {
+      const bool still = step<kSurf>(P, rng, ph, fate, info);
+      ++done;
+      if (!still || done >= P.max_steps) {
+        if (still) fate = kMaxSteps;
+        finished = true;
+        alive = false;
+      }
+    }
+    wc.add_step(act, info);
+    if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
+    if (finished) pending = true;
+  }
+  __syncthreads();
+  flush_block<true, kCount>(out, lds, snb, sns, wc);
+}
+
 // Column sums of the [blocks][words] partial histograms (and the [blocks][2]
 // sums) into the u64 tally, which must be zero on entry. blockIdx.y takes
 // every gridDim.y-th block row, so enough waves are in flight to stream the
@@ -1729,10 +1865,29 @@ void launch_gensteps_t(const DevParams& P, PhiloxKey key, uint64_t first_id,
                      out);
 }
 
+template <bool C, int S>
+void launch_gensteps_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_id,
+                            const WavefrontShape& s, const GenstepArgs& g,
+                            unsigned long long* queue, const LaunchOut& out,
+                            hipStream_t stream) {
+  hipLaunchKernelGGL((genstep_ring_kernel<C, S>), dim3(s.grid), dim3(s.block),
+                     ring_lds_bytes(out.tally.words, s.block), stream, P, key, first_id,
+                     s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g, queue, out);
+}
+
 template <bool R, bool C>
 void launch_gensteps_surf(const DevParams& P, PhiloxKey key, uint64_t first_id,
                           const WavefrontShape& s, const GenstepArgs& g,
                           unsigned long long* queue, const LaunchOut& out, hipStream_t stream) {
+  if (!R && s.stream_in_lds) {
+    switch (P.surface) {
+      case kPolished: launch_gensteps_ring_t<C, kPolished>(P, key, first_id, s, g, queue, out, stream); break;
+      case kBlack: launch_gensteps_ring_t<C, kBlack>(P, key, first_id, s, g, queue, out, stream); break;
+      case kSpecular: launch_gensteps_ring_t<C, kSpecular>(P, key, first_id, s, g, queue, out, stream); break;
+      default: launch_gensteps_ring_t<C, kLambertian>(P, key, first_id, s, g, queue, out, stream); break;
+    }
+    return;
+  }
   switch (P.surface) {
     case kPolished: launch_gensteps_t<R, C, kPolished>(P, key, first_id, s, g, queue, out, stream); break;
     case kBlack: launch_gensteps_t<R, C, kBlack>(P, key, first_id, s, g, queue, out, stream); break;
@@ -1831,7 +1986,10 @@ void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,
     launch_gensteps_surf<false, false>(P, key, first_id, shape, g, queue, out, stream);
 }
 
-int genstep_blocks_per_cu(bool records, int block, size_t lds) {
+int genstep_blocks_per_cu(bool records, int block, size_t lds, bool ring) {
+  if (!records && ring)
+    return blocks_per_cu(genstep_ring_kernel<false, kLambertian>, block,
+                         ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
   return records ? blocks_per_cu(genstep_kernel<true, false, kLambertian>, block, lds)
                  : blocks_per_cu(genstep_kernel<false, false, kLambertian>, block, lds);
 }

@@ -293,8 +293,10 @@ typedef struct optphot_genstep {
  * degenerate genstep with t0 = 0 is the point source at (x0, y0, z0), photon
  * for photon. Writes the tally (as optphot_run_gpu_tally) and, when `records`
  * is non-NULL, records of all photons at their global index (host arrays of
- * the total photon count). No counterpart in the reference, which lists
- * generation from gensteps as future work (README.md:734-735). */
+ * the total photon count). tuning->stream_in_lds selects the kernel with
+ * the Philox windows in LDS rings (tally runs only). No counterpart in the
+ * reference, which lists generation from gensteps as future work
+ * (README.md:734-735). */
 int optphot_run_gpu_gensteps(const optphot_sim_params* p, const optphot_genstep* gensteps,
                              uint32_t n_gensteps, uint64_t first_id,
                              const optphot_tally_config* c, uint64_t* tally,

@@ -119,3 +119,18 @@ def test_gpu_gensteps_equal_oracle(backend, oracle_portable, name):
         for f in FIELDS:
             assert np.array_equal(getattr(r, f).view(np.uint8), ref_r[f].view(np.uint8)), f
         assert cnt.as_dict()["emitted"] == int(g["n_photons"].sum())
+    # tally only, Philox windows in LDS rings
+    for tu in (dict(block_size=512, steps_per_refill=2, queue_chunk=256, stream_in_lds=1),
+               dict(block_size=64, steps_per_refill=1, queue_chunk=64, stream_in_lds=1),
+               dict(block_size=256, steps_per_refill=5, queue_chunk=1024, stream_in_lds=1)):
+        for counters in (True, False):
+            t, r, tm, cnt = run_gpu_gensteps(p, c, g, first_id=5, records=False, tuning=tu,
+                                             counters=counters)
+            assert r is None
+            assert np.array_equal(t.fate_counts, ref_t["fate_counts"]), tu
+            assert np.array_equal(t.time_hist, ref_t["time_hist"]), tu
+            assert np.array_equal(t.xy_hist, ref_t["xy_hist"]), tu
+            assert t.boundary_sum == ref_t["boundary_sum"]
+            assert t.scatter_sum == ref_t["scatter_sum"]
+            if counters:
+                assert cnt.as_dict()["emitted"] == int(g["n_photons"].sum())
