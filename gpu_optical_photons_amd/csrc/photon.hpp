@@ -77,8 +77,10 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c2, uint32_t c3, 
   for (int r = 0; r < 10; ++r) {
     const uint64_t p0 = static_cast<uint64_t>(x0) * 0xD2511F53u;
     const uint64_t p1 = static_cast<uint64_t>(x2) * 0xCD9E8D57u;
-    const uint32_t y0 = static_cast<uint32_t>(p1 >> 32) ^ x1 ^ k0;
-    const uint32_t y2 = static_cast<uint32_t>(p0 >> 32) ^ x3 ^ k1;
+    // hi ^ x ^ k as one gfx950 three-input bit operation (truth table 0x96 =
+    // XOR of the three), instead of two v_xor_b32
+    const uint32_t y0 = __builtin_amdgcn_bitop3_b32(static_cast<uint32_t>(p1 >> 32), x1, k0, 0x96);
+    const uint32_t y2 = __builtin_amdgcn_bitop3_b32(static_cast<uint32_t>(p0 >> 32), x3, k1, 0x96);
     x1 = static_cast<uint32_t>(p1);
     x3 = static_cast<uint32_t>(p0);
     x0 = y0;
