@@ -101,7 +101,9 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c2, uint32_t c3, 
 
 // Top 24 bits -> k * 2^-24 in (0, 1] (philox.hpp:82-84).
 __device__ __forceinline__ float to_unit(uint32_t bits) {
-  return static_cast<float>((bits >> 8) + 1u) * (1.0f / 16777216.0f);
+  // (k + 1) 2^-24 with k = bits >> 8 < 2^24, as one FMA: k 2^-24 and the sum
+  // are exact in f32, so this is the reference's value bit for bit.
+  return __builtin_fmaf(static_cast<float>(bits >> 8), 1.0f / 16777216.0f, 1.0f / 16777216.0f);
 }
 
 // A photon's stream: words [4*(blk-2), 4*blk) are held in w0..w7, `pos` is
