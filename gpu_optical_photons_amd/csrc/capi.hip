@@ -243,6 +243,28 @@ Context& ctx() {
   return c;
 }
 
+// Captures what `enqueue` launches on c.stream into an executable graph. The
+// capture is ended and the partial graph freed on every path, so a failure
+// leaves the stream usable.
+template <class F>
+hipGraphExec_t capture_graph(Context& c, F&& enqueue) {
+  HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    enqueue();
+  } catch (...) {
+    (void)hipStreamEndCapture(c.stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(c.stream, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPCHK(e);
+  return exec;
+}
+
 // ---- parameter conversion -------------------------------------------------
 
 DevParams to_dev(const optphot_sim_params& p) {
@@ -1007,13 +1029,11 @@ __attribute__((visibility("default"))) int optphot_run_gpu_tally_sorted(
     if (!c.sorted_exec || c.sorted_key != gkey) {
       if (c.sorted_exec) HIPCHK(hipGraphExecDestroy(c.sorted_exec));
       c.sorted_exec = nullptr;
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-      for (uint32_t i = 0; i < tu.iterations_per_poll; ++i) launch_sorted_iteration(P, key, A, c.stream);
-      launch_sorted_status(A, c.stream);
-      HIPCHK(hipStreamEndCapture(c.stream, &g));
-      HIPCHK(hipGraphInstantiate(&c.sorted_exec, g, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(g));
+      c.sorted_exec = capture_graph(c, [&] {
+        for (uint32_t i = 0; i < tu.iterations_per_poll; ++i)
+          launch_sorted_iteration(P, key, A, c.stream);
+        launch_sorted_status(A, c.stream);
+      });
       c.sorted_key = gkey;
     }
 
@@ -1193,14 +1213,13 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
       if (!c.batch_exec || c.batch_key != gkey) {
         if (c.batch_exec) HIPCHK(hipGraphExecDestroy(c.batch_exec));
         c.batch_exec = nullptr;
-        hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-        launch_zero_u64(scratch, kScratchSums, c.stream);  // queue head + counters
-        launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64, c.stream);
-        launch_batch_wavefront(key, first_id, n_slots, s, scratch + kScratchQueue, lo, c.stream);
-        HIPCHK(hipStreamEndCapture(c.stream, &g));
-        HIPCHK(hipGraphInstantiate(&c.batch_exec, g, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(g));
+        c.batch_exec = capture_graph(c, [&] {
+          launch_zero_u64(scratch, kScratchSums, c.stream);  // queue head + counters
+          launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64,
+                          c.stream);
+          launch_batch_wavefront(key, first_id, n_slots, s, scratch + kScratchQueue, lo,
+                                 c.stream);
+        });
         c.batch_key = gkey;
       }
       HIPCHK(hipEventRecord(c.ev[0], c.stream));
@@ -1519,22 +1538,21 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
     if (!c.step_exec || c.step_key != gkey) {
       if (c.step_exec) HIPCHK(hipGraphExecDestroy(c.step_exec));
       c.step_exec = nullptr;
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-      launch_zero_u64(scratch, kScratchSums, c.stream);  // queue head + counters
-      launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64, c.stream);
-      launch_batch_wavefront(key, first_id, n_slots, sf, scratch + kScratchQueue, lo, c.stream);
-      launch_chi2_weights(lo.evt.counts, words, bins1, a.offsets, c.step_target.as<double>(),
-                          c.step_inv_nt.as<double>(), c.step_hist.as<double>(),
-                          c.step_loss.as<double>(), c.adj_weights.as<double>(), n_sets,
-                          c.stream);
-      launch_zero_u64(a.qheads, static_cast<uint64_t>(n_sets) + 1, c.stream);
-      launch_zero_u64(reinterpret_cast<unsigned long long*>(a.grad),
-                      static_cast<uint64_t>(n_sets) * 5, c.stream);
-      launch_adjoint(table[0], surface, key, first_id, sb, a, c.stream, false);
-      HIPCHK(hipStreamEndCapture(c.stream, &g));
-      HIPCHK(hipGraphInstantiate(&c.step_exec, g, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(g));
+      c.step_exec = capture_graph(c, [&] {
+        launch_zero_u64(scratch, kScratchSums, c.stream);  // queue head + counters
+        launch_zero_u64(reinterpret_cast<unsigned long long*>(lo.evt.counts), zero_u64,
+                        c.stream);
+        launch_batch_wavefront(key, first_id, n_slots, sf, scratch + kScratchQueue, lo,
+                               c.stream);
+        launch_chi2_weights(lo.evt.counts, words, bins1, a.offsets, c.step_target.as<double>(),
+                            c.step_inv_nt.as<double>(), c.step_hist.as<double>(),
+                            c.step_loss.as<double>(), c.adj_weights.as<double>(), n_sets,
+                            c.stream);
+        launch_zero_u64(a.qheads, static_cast<uint64_t>(n_sets) + 1, c.stream);
+        launch_zero_u64(reinterpret_cast<unsigned long long*>(a.grad),
+                        static_cast<uint64_t>(n_sets) * 5, c.stream);
+        launch_adjoint(table[0], surface, key, first_id, sb, a, c.stream, false);
+      });
       c.step_key = gkey;
     }
     HIPCHK(hipEventRecord(c.ev[0], c.stream));
