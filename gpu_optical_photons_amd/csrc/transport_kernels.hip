@@ -115,6 +115,54 @@ __device__ __forceinline__ void finish(const DevParams& P, const LaunchOut& out,
   }
 }
 
+// Per-lane fate counts of one event of a batch. Every finished photon adds to
+// one of its event's five fate words; with few events (a calibration batch)
+// those words take a global atomic per photon and the atomics queue on them.
+// A lane's consecutive photons mostly come from one queue slot, i.e. one
+// event, so the lane counts in registers and adds its counts to the event's
+// row (global or LDS) when its photons move to another event, when a count
+// reaches 2^30, and at the end. Integer sums: the rows are unchanged.
+struct EventFates {
+  uint32_t e = 0xffffffffu;
+  uint32_t c0 = 0u, c1 = 0u, c2 = 0u, c3 = 0u, c4 = 0u;
+
+  __device__ __forceinline__ void flush(const DevEventTally& T) {
+    if (e == 0xffffffffu) return;
+    uint32_t* row = T.counts + static_cast<size_t>(e) * T.words;
+    if (c0) atomicAdd(row + 0, c0);
+    if (c1) atomicAdd(row + 1, c1);
+    if (c2) atomicAdd(row + 2, c2);
+    if (c3) atomicAdd(row + 3, c3);
+    if (c4) atomicAdd(row + 4, c4);
+    c0 = c1 = c2 = c3 = c4 = 0u;
+  }
+  __device__ __forceinline__ void add(const DevEventTally& T, uint32_t event, uint32_t fate) {
+    if (event != e || c0 + c1 + c2 + c3 + c4 >= (1u << 30)) {
+      flush(T);
+      e = event;
+    }
+    c0 += fate == 0u;
+    c1 += fate == 1u;
+    c2 += fate == 2u;
+    c3 += fate == 3u;
+    c4 += fate == 4u;
+  }
+};
+
+// A finished photon of event e of a batch: its fate into the lane's counts
+// (kDirect: straight into the row, for rows in LDS, where atomics are cheap),
+// and, if detected, its arrival-time bin into the event's row.
+template <bool kDirect>
+__device__ __forceinline__ void finish_event(const DevEventTally& T, EventFates& fates,
+                                             uint32_t e, const Photon& ph, uint32_t fate) {
+  uint32_t* row = T.counts + static_cast<size_t>(e) * T.words;
+  if (kDirect)
+    atomicAdd(row + fate, 1u);
+  else
+    fates.add(T, e, fate);
+  if (fate == kDetected) atomicAdd(row + 5u + time_bin(ph.t, T.time_scale, T.time_bins), 1u);
+}
+
 template <bool kTally, bool kCount>
 __device__ __forceinline__ void flush_block(const LaunchOut& out, uint32_t* lds, uint64_t snb,
                                             uint64_t sns, const WaveCounters<kCount>& wc) {
@@ -838,7 +886,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   const float* const rtab = out.evt.table[0].rayleigh;
   WaveCounters<kCount> wc;
   wc.clear();
-  uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
+  EventFates fates;
 
   uint32_t we = 0;                 // event of the wave's claimed range (uniform)
   uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
@@ -860,7 +908,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
     const bool all_dead = __ballot(alive) == 0;
     if (all_dead || it % steps_per_refill == 0) {
       if (pending) {
-        finish<false, false, true>(P, loc, nullptr, 0, ph, fate, snb, sns, pset);
+        finish_event<kLds>(loc.evt, fates, pset, ph, fate);
         pending = false;
       }
       const uint64_t dead = __ballot(!alive);
@@ -946,6 +994,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
     if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
     if (finished) pending = true;
   }
+  fates.flush(loc.evt);
   wc.flush(out.counters);
   if (kLds) {
     __syncthreads();
@@ -1238,7 +1287,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
   const float* const rtab = out.evt.table[0].rayleigh;
   WaveCounters<kCount> wc;
   wc.clear();
-  uint64_t snb = 0, sns = 0;  // unused: events tally fates and times only
+  EventFates fates;
 
   uint32_t we = 0;                 // event of the wave's claimed range (uniform)
   uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
@@ -1303,7 +1352,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
         fresh = true;
       }
       wc.add(kCntEmitted, t1 + t2);
-      if (fresh && !alive) finish<false, false, true>(P, loc, nullptr, idx, ph, fate, snb, sns, pset);
+      if (fresh && !alive) finish_event<kLds>(loc.evt, fates, pset, ph, fate);
     }
     if (__ballot(alive) == 0) {
       if (q_done) break;
@@ -1326,9 +1375,10 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
       }
       wc.add_step(act, info);
       if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
-      if (finished) finish<false, false, true>(P, loc, nullptr, idx, ph, fate, snb, sns, pset);
+      if (finished) finish_event<kLds>(loc.evt, fates, pset, ph, fate);
     }
   }
+  fates.flush(loc.evt);
   wc.flush(out.counters);
   if (kLds) {
     __syncthreads();
