@@ -1104,7 +1104,10 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     optphot_run_timing* timing) {
   return guarded([&] {
     validate_batch(params, photons, n_events, time_bins, time_max, event_tally, method);
-    optphot_wavefront_tuning tu{256, 2, 256, 0, 0, 0};
+    // queue chunk 1024: a lane then takes ~16 photons of one event in a row,
+    // so its per-event fate counts flush rarely (measured best for both the
+    // 4096-event and the 256-set calibration batches)
+    optphot_wavefront_tuning tu{256, 2, 1024, 0, 0, 0};
     if (tuning) {
       if (tuning->block_size) tu.block_size = tuning->block_size;
       if (tuning->steps_per_refill) tu.steps_per_refill = tuning->steps_per_refill;
@@ -1407,7 +1410,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
                               "steps_per_refill >= 1");
       return d;
     };
-    const optphot_wavefront_tuning tf = merge(forward_tuning, {256, 2, 256, 0, 0, 0});
+    const optphot_wavefront_tuning tf = merge(forward_tuning, {256, 2, 1024, 0, 0, 0});
     const optphot_wavefront_tuning tb = merge(adjoint_tuning, {256, 2, 256, 0, 0, 0});
     Context& c = ctx();
     std::lock_guard<std::mutex> lock(c.mu);
