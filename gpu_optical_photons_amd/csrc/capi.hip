@@ -292,6 +292,12 @@ DevParams to_dev(const optphot_sim_params& p) {
   d.surface = p.surface;
   d.source = p.source;
   d.max_steps = p.max_steps;
+  auto in = [](float v, float lo, float hi) { return v >= lo && v <= hi; };
+  const bool box = in(p.hx, 0x1p-60f, 0x1p40f) && in(p.hy, 0x1p-60f, 0x1p40f) &&
+                   in(p.hz, 0x1p-60f, 0x1p40f);
+  const bool idx = in(p.n_scint, 0x1p-20f, 0x1p20f) && in(p.n_det, 0x1p-20f, 0x1p20f) &&
+                   in(p.n_out, 0x1p-20f, 0x1p20f);
+  d.fast_div = (box ? 1u : 0u) | (idx ? 2u : 0u);
   d.rayleigh = ctx().rayleigh.as<const float>();
   return d;
 }

@@ -55,6 +55,10 @@ struct DevParams {
   float tau;
   uint32_t det_matched, out_matched;  // n_scint == n_det, n_scint == n_out
   uint32_t surface, source, max_steps;
+  // Operand ranges that make the short divisions (div_n) exact, set on the
+  // host: bit 0 = every box half size in [2^-60, 2^40] (box exit and hit-map
+  // bins), bit 1 = n_scint, n_det and n_out in [2^-20, 2^20] (Fresnel).
+  uint32_t fast_div;
   // Rayleigh cosines of every 24-bit uniform (2^24 entries, entry k for
   // u = (k + 1) 2^-24, filled on the device by rayleigh_cos); nullptr: the
   // step computes them.
@@ -349,11 +353,16 @@ __device__ __forceinline__ uint32_t time_bin(float t, float time_scale, uint32_t
   return (f >= static_cast<float>(bins)) ? bins : static_cast<uint32_t>(f);
 }
 
+// With hx, hy in [2^-60, 2^40] (fast), x + hx is 0 or at least half an ulp
+// of hx (x lies on the readout face, within [-hx, hx]) and 2 hx is normal, so
+// div_n gives the IEEE quotients.
 __device__ __forceinline__ uint32_t xy_bin(float x, float y, float hx, float hy, float two_hx,
-                                           float two_hy, uint32_t nb_bins) {
+                                           float two_hy, uint32_t nb_bins, bool fast) {
   const float nb = static_cast<float>(nb_bins);
-  const uint32_t ix = static_cast<uint32_t>(__builtin_fminf((x + hx) / two_hx * nb, nb - 1.0f));
-  const uint32_t iy = static_cast<uint32_t>(__builtin_fminf((y + hy) / two_hy * nb, nb - 1.0f));
+  const float fx = fast ? div_n(x + hx, two_hx) : (x + hx) / two_hx;
+  const float fy = fast ? div_n(y + hy, two_hy) : (y + hy) / two_hy;
+  const uint32_t ix = static_cast<uint32_t>(__builtin_fminf(fx * nb, nb - 1.0f));
+  const uint32_t iy = static_cast<uint32_t>(__builtin_fminf(fy * nb, nb - 1.0f));
   return iy * nb_bins + ix;
 }
 
@@ -526,6 +535,17 @@ __device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenste
 // fused step() below and the per-interaction kernels, which run them in
 // separate launches on compacted queues.
 
+// (±h - p) / d for the box exit. With the box half sizes in [2^-60, 2^40]
+// (P.fast_div bit 0) the numerator is 0 or between 2^-86 and 2^41 in
+// magnitude (p lies in the box, so a nonzero ±h - p is at least half an ulp
+// of h), and with |d| >= 2^-60 the quotient is normal: div_n then gives the
+// IEEE quotient. A component below 2^-60, or a box outside that range, takes
+// the full division.
+__device__ __forceinline__ float box_div(const DevParams& P, float num, float d) {
+  if ((P.fast_div & 1u) && __builtin_fabsf(d) >= 0x1p-60f) return div_n(num, d);
+  return num / d;
+}
+
 // Where the step ends: absorbed in the bulk, scattered, or on face `face`
 // (axis = face / 2, plus = odd face); cos_i = dot(d, outward normal) at a
 // boundary.
@@ -552,7 +572,8 @@ __device__ __forceinline__ Advance advance(const DevParams& P, const RNG& rng, P
   int face = -1;
   if (ph.d.x != 0.0f) {
     const bool pos = ph.d.x > 0.0f;
-    const float tt = ((pos ? P.hx : -P.hx) - ph.p.x) / ph.d.x;
+    const float num = (pos ? P.hx : -P.hx) - ph.p.x;
+    const float tt = box_div(P, num, ph.d.x);
     if (tt < dist) {
       dist = tt;
       face = pos ? 1 : 0;
@@ -560,7 +581,8 @@ __device__ __forceinline__ Advance advance(const DevParams& P, const RNG& rng, P
   }
   if (ph.d.y != 0.0f) {
     const bool pos = ph.d.y > 0.0f;
-    const float tt = ((pos ? P.hy : -P.hy) - ph.p.y) / ph.d.y;
+    const float num = (pos ? P.hy : -P.hy) - ph.p.y;
+    const float tt = box_div(P, num, ph.d.y);
     if (tt < dist) {
       dist = tt;
       face = pos ? 3 : 2;
@@ -568,7 +590,8 @@ __device__ __forceinline__ Advance advance(const DevParams& P, const RNG& rng, P
   }
   if (ph.d.z != 0.0f) {
     const bool pos = ph.d.z > 0.0f;
-    const float tt = ((pos ? P.hz : -P.hz) - ph.p.z) / ph.d.z;
+    const float num = (pos ? P.hz : -P.hz) - ph.p.z;
+    const float tt = box_div(P, num, ph.d.z);
     if (tt < dist) {
       dist = tt;
       face = pos ? 5 : 4;
@@ -620,8 +643,16 @@ __device__ __forceinline__ float fresnel_prob(const DevParams& P, bool readout, 
   // 1 - sin2_t is in (0, 1] and at least 2^-24 unless total internal
   // reflection (sin2_t >= 1), where R is not used
   const float cos_t = sqrt_n(1.0f - sin2_t);
-  const float rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
-  const float rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t);
+  // With the indices in [2^-20, 2^20] (P.fast_div bit 1) and R in use (no
+  // total internal reflection, n1 != n2), cos_i or cos_t is at least
+  // sqrt(|1 - (n1/n2)^2|) >= 2^-12, so both denominators exceed 2^-32 and the
+  // numerators are 0 or above 2^-56: div_n gives the IEEE quotients. When R
+  // is not used its value does not matter.
+  const float a_s = n1 * cos_i - n2 * cos_t, b_s = n1 * cos_i + n2 * cos_t;
+  const float a_p = n2 * cos_i - n1 * cos_t, b_p = n2 * cos_i + n1 * cos_t;
+  const bool fast = P.fast_div & 2u;
+  const float rs = fast ? div_n(a_s, b_s) : a_s / b_s;
+  const float rp = fast ? div_n(a_p, b_p) : a_p / b_p;
   const float R = 0.5f * (rs * rs + rp * rp);
   const bool tir = sin2_t >= 1.0f;
   scored = !matched && !tir;

@@ -85,8 +85,9 @@ __device__ __forceinline__ void tally_photon(const DevParams& P, const DevTally&
   atomicAdd(&lds[fate], 1u);
   if (fate == kDetected) {
     atomicAdd(&lds[5u + time_bin(t, T.time_scale, T.time_bins)], 1u);
-    atomicAdd(&lds[6u + T.time_bins + xy_bin(x, y, P.hx, P.hy, T.two_hx, T.two_hy, T.xy_bins)],
-              1u);
+    const uint32_t xy =
+        xy_bin(x, y, P.hx, P.hy, T.two_hx, T.two_hy, T.xy_bins, P.fast_div & 1u);
+    atomicAdd(&lds[6u + T.time_bins + xy], 1u);
   }
 }
 

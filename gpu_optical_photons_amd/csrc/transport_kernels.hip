@@ -106,9 +106,9 @@ __device__ __forceinline__ void finish(const DevParams& P, const LaunchOut& out,
     atomicAdd(&lds[fate], 1u);
     if (fate == kDetected) {
       atomicAdd(&lds[5u + time_bin(ph.t, T.time_scale, T.time_bins)], 1u);
-      atomicAdd(&lds[6u + T.time_bins + xy_bin(ph.p.x, ph.p.y, P.hx, P.hy, T.two_hx, T.two_hy,
-                                               T.xy_bins)],
-                1u);
+      const uint32_t xy = xy_bin(ph.p.x, ph.p.y, P.hx, P.hy, T.two_hx, T.two_hy, T.xy_bins,
+                                 P.fast_div & 1u);
+      atomicAdd(&lds[6u + T.time_bins + xy], 1u);
     }
     snb += ph.nb;
     sns += ph.ns;

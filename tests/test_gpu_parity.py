@@ -175,3 +175,33 @@ def test_rayleigh_table_equals_oracle(backend, oracle_portable):
     backend._check(fn(dev.ctypes.data, dev.size))
     ref = oracle_portable.rayleigh_cos_table()
     np.testing.assert_array_equal(dev.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["tiny_box", "huge_box", "huge_index"])
+def test_full_division_fallbacks(backend, oracle_portable, case):
+    """The kernels divide with a short exact sequence only where their
+    operands are in range (box half sizes in [2^-60, 2^40], refractive indices
+    in [2^-20, 2^20]); outside, the IEEE divisions run. Both must give the
+    oracle's records and tally."""
+    from gpu_optical_photons_amd.params import SimParams, Source, Surface, TallyConfig
+    p = {
+        "tiny_box": SimParams(hx=2e-19, hy=3e-19, hz=1e-19, abs_length=4e-18, scat_length=5e-19,
+                              surface=Surface.LAMBERTIAN, reflectivity=0.9, n_det=1.5,
+                              source=Source.VOLUME, tau=0.0, seed=21),
+        "huge_box": SimParams(hx=2e12, hy=3e12, hz=1.5e12, abs_length=3e13, scat_length=5e12,
+                              surface=Surface.POLISHED, n_det=1.5, source=Source.VOLUME,
+                              tau=4.0, seed=22),
+        "huge_index": SimParams(n_det=3e6, n_out=1.0, abs_length=500.0, scat_length=200.0,
+                                surface=Surface.POLISHED, seed=23),
+    }[case]
+    n = 20_000
+    ref = oracle_portable.records(p, n)
+    gpu, _ = backend.run_gpu_records(p, n)
+    assert_records_identical(gpu, ref, case)
+    c = TallyConfig()
+    rt = oracle_portable.tally(p, n, c.time_bins, c.time_max, c.xy_bins)
+    t, _, _, _ = backend.run_gpu_tally_wavefront(
+        p, c, n, tuning=dict(block_size=512, steps_per_refill=2, queue_chunk=512, stream_in_lds=1))
+    np.testing.assert_array_equal(t.fate_counts, rt["fate_counts"])
+    np.testing.assert_array_equal(t.time_hist, rt["time_hist"])
+    np.testing.assert_array_equal(t.xy_hist, rt["xy_hist"])
