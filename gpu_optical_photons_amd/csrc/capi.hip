@@ -295,8 +295,11 @@ DevParams to_dev(const optphot_sim_params& p) {
   auto in = [](float v, float lo, float hi) { return v >= lo && v <= hi; };
   const bool box = in(p.hx, 0x1p-60f, 0x1p40f) && in(p.hy, 0x1p-60f, 0x1p40f) &&
                    in(p.hz, 0x1p-60f, 0x1p40f);
+  // an unmatched pair must also have eta^2 != 1 (two distinct floats can
+  // have a ratio that rounds to 1), so that |1 - eta^2| >= 2^-24
   const bool idx = in(p.n_scint, 0x1p-20f, 0x1p20f) && in(p.n_det, 0x1p-20f, 0x1p20f) &&
-                   in(p.n_out, 0x1p-20f, 0x1p20f);
+                   in(p.n_out, 0x1p-20f, 0x1p20f) && (d.det_matched || d.eta2_det != 1.0f) &&
+                   (d.out_matched || d.eta2_out != 1.0f);
   d.fast_div = (box ? 1u : 0u) | (idx ? 2u : 0u);
   d.rayleigh = ctx().rayleigh.as<const float>();
   return d;

@@ -57,7 +57,8 @@ struct DevParams {
   uint32_t surface, source, max_steps;
   // Operand ranges that make the short divisions (div_n) exact, set on the
   // host: bit 0 = every box half size in [2^-60, 2^40] (box exit and hit-map
-  // bins), bit 1 = n_scint, n_det and n_out in [2^-20, 2^20] (Fresnel).
+  // bins), bit 1 = n_scint, n_det and n_out in [2^-20, 2^20] and eta^2 != 1
+  // for each unmatched pair (Fresnel).
   uint32_t fast_div;
   // Rayleigh cosines of every 24-bit uniform (2^24 entries, entry k for
   // u = (k + 1) 2^-24, filled on the device by rayleigh_cos); nullptr: the
@@ -643,11 +644,12 @@ __device__ __forceinline__ float fresnel_prob(const DevParams& P, bool readout, 
   // 1 - sin2_t is in (0, 1] and at least 2^-24 unless total internal
   // reflection (sin2_t >= 1), where R is not used
   const float cos_t = sqrt_n(1.0f - sin2_t);
-  // With the indices in [2^-20, 2^20] (P.fast_div bit 1) and R in use (no
-  // total internal reflection, n1 != n2), cos_i or cos_t is at least
-  // sqrt(|1 - (n1/n2)^2|) >= 2^-12, so both denominators exceed 2^-32 and the
-  // numerators are 0 or above 2^-56: div_n gives the IEEE quotients. When R
-  // is not used its value does not matter.
+  // With the indices in [2^-20, 2^20] and eta^2 != 1 unless matched
+  // (P.fast_div bit 1), |1 - eta^2| >= 2^-24. When R is in use (no total
+  // internal reflection, not matched), cos_i (eta^2 > 1) or cos_t
+  // (eta^2 < 1) is then at least about 2^-12.5, so both denominators exceed
+  // 2^-33 and the numerators are 0 or above 2^-58: div_n gives the IEEE
+  // quotients. When R is not used its value does not matter.
   const float a_s = n1 * cos_i - n2 * cos_t, b_s = n1 * cos_i + n2 * cos_t;
   const float a_p = n2 * cos_i - n1 * cos_t, b_p = n2 * cos_i + n1 * cos_t;
   const bool fast = P.fast_div & 2u;

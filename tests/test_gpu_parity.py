@@ -177,12 +177,14 @@ def test_rayleigh_table_equals_oracle(backend, oracle_portable):
     np.testing.assert_array_equal(dev.view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("case", ["tiny_box", "huge_box", "huge_index"])
-def test_full_division_fallbacks(backend, oracle_portable, case):
+@pytest.mark.parametrize("case", ["tiny_box", "huge_box", "huge_index", "near_matched"])
+def test_division_paths_at_range_limits(backend, oracle_portable, case):
     """The kernels divide with a short exact sequence only where their
     operands are in range (box half sizes in [2^-60, 2^40], refractive indices
-    in [2^-20, 2^20]); outside, the IEEE divisions run. Both must give the
-    oracle's records and tally."""
+    in [2^-20, 2^20] with (n1/n2)^2 != 1 unless matched); outside, the IEEE
+    divisions run. Boxes and indices beyond the limits, and indices one ulp
+    apart (the edge of the Fresnel argument), must all give the oracle's
+    records and tally."""
     from gpu_optical_photons_amd.params import SimParams, Source, Surface, TallyConfig
     p = {
         "tiny_box": SimParams(hx=2e-19, hy=3e-19, hz=1e-19, abs_length=4e-18, scat_length=5e-19,
@@ -193,6 +195,10 @@ def test_full_division_fallbacks(backend, oracle_portable, case):
                               tau=4.0, seed=22),
         "huge_index": SimParams(n_det=3e6, n_out=1.0, abs_length=500.0, scat_length=200.0,
                                 surface=Surface.POLISHED, seed=23),
+        "near_matched": SimParams(n_det=float(np.nextafter(np.float32(1.63), np.float32(2))),
+                                  n_out=float(np.nextafter(np.float32(1.63), np.float32(0))),
+                                  abs_length=500.0, scat_length=200.0, surface=Surface.POLISHED,
+                                  seed=24),
     }[case]
     n = 20_000
     ref = oracle_portable.records(p, n)
