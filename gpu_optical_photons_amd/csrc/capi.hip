@@ -208,6 +208,11 @@ struct Context {
     max_lds = prop.sharedMemPerBlock;
     name = std::string(prop.name) + " (" + prop.gcnArchName + ", " + std::to_string(cu_count) +
            " CUs)";
+    // The ring kernels address their LDS ring by OR-ing slot bits into
+    // addresses that start at LDS 0; static LDS would move that start.
+    if (!ring_kernels_start_at_lds_zero())
+      throw std::runtime_error("LDS-ring kernels were built with static LDS; their ring "
+                               "addressing needs the dynamic LDS to start at address 0");
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));

@@ -206,8 +206,8 @@ void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, 
 // kBatchLdsMax bytes; batch_lds_bytes() is that size, or 0 for global atomics.
 constexpr size_t kBatchLdsMax = 32768;
 size_t batch_lds_bytes(uint32_t n_events, uint32_t words);
-// LDS bytes of a batch launch with ring streams: the event tallies (if any),
-// then 16 ring words per lane.
+// LDS bytes of a batch launch with ring streams: the ring region
+// (ring_words), then the event tallies (if any).
 size_t batch_ring_lds_bytes(size_t tally_lds, int block);
 int batch_blocks_per_cu(int block, size_t lds, bool ring = false);
 
@@ -257,9 +257,19 @@ void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
 
 // Occupancy queries (blocks per CU for a block size and dynamic LDS bytes).
 int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);
-// LDS bytes of a wavefront tally launch with the streams in LDS: the tally
-// words, then 16 ring words per lane.
+// Words of the LDS ring region of a block: 16 slots of the block size rounded
+// up to a power of two (the device side's ring_region_words()).
+inline size_t ring_words(int block) {
+  size_t p = 1;
+  while (p < static_cast<size_t>(block)) p <<= 1;
+  return 16u * p;
+}
+// LDS bytes of a wavefront tally launch with the streams in LDS: the ring
+// region and the tally words.
 size_t ring_lds_bytes(uint32_t tally_words, int block);
+// Whether every kernel that places its ring region at LDS address 0 has no
+// static LDS, so that its dynamic LDS (and the region) starts at address 0.
+bool ring_kernels_start_at_lds_zero();
 // LDS bytes of the sorted-phase kernel: the ring layout, then the per-parity
 // queue counters and 9 words per lane of queues and exchange slots.
 size_t phase_lds_bytes(uint32_t tally_words, int block);

@@ -158,23 +158,62 @@ struct Stream {
 };
 
 // The same stream with its window in LDS: a ring of 16 words per lane,
-// slot-major (slot j of lane l is ring_base[j * stride + l], stride = the
-// block size), so a wave's 64 accesses to one slot index fall in distinct
-// banks. `pos` counts the words consumed and `wr` the words generated (4 per
-// Philox block); words pos .. wr-1 are readable and wr - pos <= 16. Reading
-// a word is one LDS load at a computed address instead of a select chain
-// over registers, and a lane with room can take a block whenever the wave
-// runs a generation pass, so passes are shared out over many steps.
+// slot-major (slot j of lane l at byte j * 2^shift + 4 l of the ring region,
+// 2^shift = 4 x the block size rounded up to a power of two), so a wave's 64
+// accesses to one slot index fall in distinct banks. `pos` counts the words
+// consumed and `wr` the words generated (4 per Philox block); words
+// pos .. wr-1 are readable and wr - pos <= 16. Reading a word is one LDS load
+// at a computed address instead of a select chain over registers, and a lane
+// with room can take a block whenever the wave runs a generation pass, so
+// passes are shared out over many steps.
+//
+// With kAligned the region's LDS byte address is a multiple of its size
+// (16 << shift), so the slot bits of a lane's address are zero and the
+// address of word w is ((w << shift) & slot mask) | lane: two instructions
+// (v_add_lshl, v_and_or). Otherwise the lane's address is added (three).
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
-struct RingStream {
-  lds_u32* ring;  // this lane's slot 0
-  uint32_t stride;
+constexpr uint32_t kRingSlots = 16u;
+
+// log2 of the slot stride in bytes for the running block.
+__device__ __forceinline__ uint32_t ring_shift() {
+  const uint32_t b = blockDim.x;
+  return 2u + (b > 1u ? 32u - static_cast<uint32_t>(__clz(b - 1u)) : 0u);
+}
+
+// 32-bit words of the ring region of the running block.
+__device__ __forceinline__ uint32_t ring_region_words() { return kRingSlots << (ring_shift() - 2u); }
+
+template <bool kAligned>
+struct RingStreamT {
+  uint32_t lane;   // LDS byte address of this lane's slot 0
+  uint32_t shift;  // log2 of the slot stride in bytes (uniform)
+  uint32_t mask;   // the slot bits of an address, (kSlots - 1) << shift (uniform)
   uint32_t pos, wr;
   uint32_t id_lo, id_hi;
 
-  static constexpr uint32_t kSlots = 16u;
+  static constexpr uint32_t kSlots = kRingSlots;
 
+  // `region`: kSlots << ring_shift() bytes of LDS (aligned as above when
+  // kAligned); the stream starts empty.
+  __device__ __forceinline__ void place(const uint32_t* region) {
+    shift = ring_shift();
+    mask = (kSlots - 1u) << shift;
+    // kept opaque: seen through, the compiler rewrites (w << shift) & mask as
+    // (w & 15) << shift, one instruction more per access
+    asm volatile("" : "+s"(mask));
+    lane = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((const lds_u32*)region)) +
+           4u * threadIdx.x;
+    pos = wr = 0u;
+    id_lo = id_hi = 0u;
+  }
+  __device__ __forceinline__ uint32_t addr(uint32_t word) const {
+    const uint32_t slot = (word << shift) & mask;
+    return kAligned ? (slot | lane) : slot + lane;
+  }
+  __device__ __forceinline__ static lds_u32* at(uint32_t a) {
+    return reinterpret_cast<lds_u32*>(static_cast<uintptr_t>(a));
+  }
   __device__ __forceinline__ void reset(uint64_t photon_id) {
     id_lo = static_cast<uint32_t>(photon_id);
     id_hi = static_cast<uint32_t>(photon_id >> 32);
@@ -183,23 +222,25 @@ struct RingStream {
   }
   __device__ __forceinline__ uint32_t level() const { return wr - pos; }
   __device__ __forceinline__ bool room() const { return wr - pos <= kSlots - 4u; }
-  // Appends the next Philox block (counter wr / 4) to the ring.
+  // Appends the next Philox block (counter wr / 4) to the ring; its four
+  // slots are consecutive (wr is a multiple of 4).
   __device__ __forceinline__ void generate(PhiloxKey key) {
     uint32_t a, b, c, d;
     philox10(wr >> 2, id_lo, id_hi, key, a, b, c, d);
-    const uint32_t sl = wr & (kSlots - 1u);
-    ring[sl * stride] = a;
-    ring[(sl + 1u) * stride] = b;
-    ring[(sl + 2u) * stride] = c;
-    ring[(sl + 3u) * stride] = d;
+    const uint32_t p = addr(wr), s = 1u << shift;
+    *at(p) = a;
+    *at(p + s) = b;
+    *at(p + 2u * s) = c;
+    *at(p + 3u * s) = d;
     wr += 4u;
   }
   // Word pos + K (the caller keeps pos + K < wr).
   template <int K>
   __device__ __forceinline__ uint32_t peek() const {
-    return ring[((pos + K) & (kSlots - 1u)) * stride];
+    return *at(addr(pos + K));
   }
 };
+typedef RingStreamT<true> RingStream;
 
 // Correctly rounded a / b and sqrt(x) for operands in a normal range, where
 // the compiler's full IEEE sequences spend instructions on range scaling and

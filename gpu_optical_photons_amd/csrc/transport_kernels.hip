@@ -481,8 +481,10 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
                                                               LaunchOut out) {
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
-  const uint32_t ring_off = (out.tally.words + 3u) & ~3u;
-  zero_lds(lds, out.tally.words);
+  // LDS: the ring region first (at address 0, a multiple of its size), then
+  // the block's tally
+  uint32_t* const tal = lds + ring_region_words();
+  zero_lds(tal, out.tally.words);
   __syncthreads();
   const uint32_t lane = lane_id();
   WaveCounters<kCount> wc;
@@ -496,11 +498,7 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
   bool pending = false;  // a finished photon not yet tallied
   uint32_t done = 0, fate = kMaxSteps;
   RingStream rng;
-  rng.ring = (lds_u32*)(lds + ring_off) + threadIdx.x;
-  rng.stride = blockDim.x;
-  rng.pos = 0u;
-  rng.wr = 0u;
-  rng.id_lo = rng.id_hi = 0u;
+  rng.place(lds);
   Photon ph;
   const uint32_t emit_words = P.source == kVolume ? 6u : 3u;
 
@@ -509,7 +507,7 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
     const bool all_dead = __ballot(alive) == 0;
     if (all_dead || it % steps_per_refill == 0) {
       if (pending) {
-        finish<false, true>(P, out, lds, 0, ph, fate, snb, sns);
+        finish<false, true>(P, out, tal, 0, ph, fate, snb, sns);
         pending = false;
       }
       const uint64_t dead = __ballot(!alive);
@@ -584,7 +582,7 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
     if (finished) pending = true;
   }
   __syncthreads();
-  flush_block<true, kCount>(out, lds, snb, sns, wc);
+  flush_block<true, kCount>(out, tal, snb, sns, wc);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,7 +621,7 @@ __global__ __launch_bounds__(1024) void wavefront_phase_kernel(DevParams P0, Phi
   const uint32_t B = blockDim.x;
   const uint32_t tid = threadIdx.x;
   const uint32_t ring_off = (out.tally.words + 3u) & ~3u;
-  uint32_t* cnt = lds + ring_off + RingStream::kSlots * B;  // [2 parities][4]
+  uint32_t* cnt = lds + ring_off + ring_region_words();  // [2 parities][4]
   uint32_t* rot_k = cnt + 8;  // owner | code << 16
   uint32_t* rot_c = rot_k + B;
   uint32_t* rot_p = rot_c + B;
@@ -645,12 +643,10 @@ __global__ __launch_bounds__(1024) void wavefront_phase_kernel(DevParams P0, Phi
   bool q_done = false;
   bool alive = false, fresh = false, pending = false;
   uint32_t done = 0, fate = kMaxSteps;
-  RingStream rng;
-  rng.ring = (lds_u32*)(lds + ring_off) + tid;
-  rng.stride = B;
-  rng.pos = 0u;
-  rng.wr = 0u;
-  rng.id_lo = rng.id_hi = 0u;
+  // the ring follows the tally, at an address that is not a multiple of its
+  // size: the unaligned addressing
+  RingStreamT<false> rng;
+  rng.place(lds + ring_off);
   Photon ph;
   const uint32_t emit_words = P.source == kVolume ? 6u : 3u;
   const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
@@ -871,13 +867,16 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
                                                           unsigned long long* queue,
                                                           LaunchOut out) {
   extern __shared__ uint32_t lds[];
+  // LDS: the ring region first (at address 0, a multiple of its size), then
+  // the block's event tallies (kLds)
+  uint32_t* const evl = lds + ring_region_words();
   const uint32_t lds_words = kLds ? out.evt.n_events * out.evt.words : 0u;
   if (kLds) {
-    zero_lds(lds, lds_words);
+    zero_lds(evl, lds_words);
     __syncthreads();
   }
   LaunchOut loc = out;
-  if (kLds) loc.evt.counts = lds;
+  if (kLds) loc.evt.counts = evl;
   const uint32_t lane = lane_id();
   const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
@@ -897,11 +896,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   uint32_t done = 0, fate = kMaxSteps;
   DevParams P{};
   RingStream rng;
-  rng.ring = (lds_u32*)(lds + ((lds_words + 3u) & ~3u)) + threadIdx.x;
-  rng.stride = blockDim.x;
-  rng.pos = 0u;
-  rng.wr = 0u;
-  rng.id_lo = rng.id_hi = 0u;
+  rng.place(lds);
   Photon ph;
 
   for (uint32_t it = 0;; ++it) {
@@ -999,7 +994,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   if (kLds) {
     __syncthreads();
     for (uint32_t w = threadIdx.x; w < lds_words; w += blockDim.x)
-      if (lds[w]) atomicAdd(out.evt.counts + w, lds[w]);
+      if (evl[w]) atomicAdd(out.evt.counts + w, evl[w]);
   }
 }
 
@@ -1126,8 +1121,10 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
                                                             LaunchOut out) {
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
-  const uint32_t ring_off = (out.tally.words + 3u) & ~3u;
-  zero_lds(lds, out.tally.words);
+  // LDS: the ring region first (at address 0, a multiple of its size), then
+  // the block's tally
+  uint32_t* const tal = lds + ring_region_words();
+  zero_lds(tal, out.tally.words);
   __syncthreads();
   const uint32_t lane = lane_id();
   WaveCounters<kCount> wc;
@@ -1142,11 +1139,7 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
   uint32_t lg = 0;  // the lane's genstep, from its refill to its emission
   uint32_t done = 0, fate = kMaxSteps;
   RingStream rng;
-  rng.ring = (lds_u32*)(lds + ring_off) + threadIdx.x;
-  rng.stride = blockDim.x;
-  rng.pos = 0u;
-  rng.wr = 0u;
-  rng.id_lo = rng.id_hi = 0u;
+  rng.place(lds);
   Photon ph;
   constexpr uint32_t kEmitWords = 4u;  // u along the segment (if any), cos, phi, delay
 
@@ -1155,7 +1148,7 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
     const bool all_dead = __ballot(alive) == 0;
     if (all_dead || it % steps_per_refill == 0) {
       if (pending) {
-        finish<false, true>(P, out, lds, 0, ph, fate, snb, sns);
+        finish<false, true>(P, out, tal, 0, ph, fate, snb, sns);
         pending = false;
       }
       const uint64_t dead = __ballot(!alive);
@@ -1244,7 +1237,7 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
     if (finished) pending = true;
   }
   __syncthreads();
-  flush_block<true, kCount>(out, lds, snb, sns, wc);
+  flush_block<true, kCount>(out, tal, snb, sns, wc);
 }
 
 // Column sums of the [blocks][words] partial histograms (and the [blocks][2]
@@ -1801,8 +1794,38 @@ int blocks_per_cu(K kernel, int block, size_t lds) {
 }  // namespace
 
 size_t ring_lds_bytes(uint32_t tally_words, int block) {
-  return (((static_cast<size_t>(tally_words) + 3u) & ~size_t{3}) +
-          static_cast<size_t>(RingStream::kSlots) * block) * sizeof(uint32_t);
+  return (((static_cast<size_t>(tally_words) + 3u) & ~size_t{3}) + ring_words(block)) *
+         sizeof(uint32_t);
+}
+
+bool ring_kernels_start_at_lds_zero() {
+  const void* kernels[] = {
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kPolished>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kBlack>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kSpecular>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kLambertian>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kPolished>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kBlack>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kSpecular>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kLambertian>),
+      reinterpret_cast<const void*>(&batch_ring_kernel<false, false>),
+      reinterpret_cast<const void*>(&batch_ring_kernel<false, true>),
+      reinterpret_cast<const void*>(&batch_ring_kernel<true, false>),
+      reinterpret_cast<const void*>(&batch_ring_kernel<true, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kPolished>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kBlack>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kSpecular>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kLambertian>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kPolished>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kBlack>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kSpecular>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kLambertian>),
+  };
+  for (const void* k : kernels) {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, k) != hipSuccess || a.sharedSizeBytes != 0) return false;
+  }
+  return true;
 }
 
 size_t phase_lds_bytes(uint32_t tally_words, int block) {
@@ -2018,8 +2041,7 @@ size_t batch_lds_bytes(uint32_t n_events, uint32_t words) {
 }
 
 size_t batch_ring_lds_bytes(size_t tally_lds, int block) {
-  return ((tally_lds + 15u) & ~size_t{15}) +
-         static_cast<size_t>(RingStream::kSlots) * block * sizeof(uint32_t);
+  return ring_words(block) * sizeof(uint32_t) + tally_lds;
 }
 
 void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,
