@@ -167,7 +167,7 @@ struct Context {
   std::vector<hipEvent_t> piece_ev;  // records runs: launch start / end, copy done
   std::unique_ptr<CopyPool> pool;
   DevBuf partials, scratch, tally;
-  DevBuf batch_table, batch_offsets, batch_counts, batch_order;
+  DevBuf batch_table, batch_offsets, batch_counts, batch_slots;
   DevBuf rayleigh;  // 2^24 Rayleigh cosines (DevParams::rayleigh), built at init
   DevBuf adj_weights, adj_grad, adj_queue;
   // Captured graph of the batched launch and the arguments it was built for.
@@ -698,6 +698,102 @@ void validate_batch(const optphot_sim_params* params, const uint64_t* photons, u
   }
 }
 
+// Expected photon history length of an event, in steps up to a constant
+// factor: the mean (`mean`) and the longest of its `n` photons (`longest`).
+// They only order the batch queue. A photon survives a wall with probability
+// w (the reflectivity of specular and Lambertian walls; 1 for polished walls,
+// where total internal reflection keeps the photons that make long
+// histories; 0 for black walls) and the bulk from wall to wall with
+// probability exp(-m / L), m = 4 V / S the mean chord of the box, L the
+// absorption length (and, for polished walls, the scattering length, since a
+// scatter lets a trapped photon out). A history then has about 1 / (1 - s)
+// steps, s = w exp(-m / L), and the longest of n about ln(n) times that,
+// capped at max_steps.
+struct HistoryEstimate {
+  double mean, longest;
+};
+HistoryEstimate history_estimate(const optphot_sim_params& p, uint64_t n) {
+  const double hx = p.hx, hy = p.hy, hz = p.hz;
+  const double area = hx * hy + hy * hz + hz * hx;
+  const double chord = area > 0.0 ? 4.0 * hx * hy * hz / area : 0.0;
+  double att = 0.0;
+  if (std::isfinite(p.abs_length) && p.abs_length > 0.0f) att += chord / p.abs_length;
+  if (p.surface == kPolished && std::isfinite(p.scat_length) && p.scat_length > 0.0f)
+    att += chord / p.scat_length;
+  const double wall = p.surface == kBlack      ? 0.0
+                      : p.surface == kPolished ? 1.0
+                                               : std::min(1.0, std::max(0.0, double(p.reflectivity)));
+  const double s = std::min(wall * std::exp(-att), 1.0 - 1e-9);
+  const double cap = static_cast<double>(p.max_steps);
+  HistoryEstimate h{std::min(1.0 / (1.0 - s), cap),
+                    std::min(std::log(static_cast<double>(n) + 1.0) / (1.0 - s), cap)};
+  if (!std::isfinite(h.mean)) h.mean = 0.0;
+  if (!std::isfinite(h.longest)) h.longest = 0.0;
+  return h;
+}
+
+// The batch queue: one slot (event << 32 | chunk index) per chunk of `chunk`
+// photons of every event, for a launch of `lanes` lanes. Chunk j of every
+// event precedes chunk j + 1, so the waves in flight work on many events at
+// once and the per-event tally atomics spread over many rows; the events are
+// grouped by branch structure (surface model, scattering on, absorption on),
+// so the consecutive slots a wave claims mostly share it.
+// A persistent launch lasts at least as long as its longest history. When an
+// event's longest history is expected to reach a fifth of the steps each lane
+// makes on average, that history would stretch the end of the launch if it
+// started late: such events then form a first part of the queue, laid out
+// the same way with the longest first in each group, so that their long
+// histories run alongside the bulk.
+std::vector<unsigned long long> batch_slots(const optphot_sim_params* params,
+                                            const uint64_t* photons, uint32_t n,
+                                            uint32_t chunk, uint64_t lanes) {
+  std::vector<HistoryEstimate> est(n);
+  double steps = 0.0;
+  for (uint32_t e = 0; e < n; ++e) {
+    est[e] = history_estimate(params[e], photons[e]);
+    steps += est[e].mean * static_cast<double>(photons[e]);
+  }
+  const double per_lane = steps / static_cast<double>(std::max<uint64_t>(1, lanes));
+  auto long_first = [&](uint32_t e) { return est[e].longest >= 0.2 * per_lane; };
+  auto group = [&](uint32_t e) {
+    return params[e].surface * 4u + (params[e].scat_length != kInf ? 2u : 0u) +
+           (params[e].abs_length != kInf ? 1u : 0u);
+  };
+  std::vector<uint32_t> order(n);
+  for (uint32_t e = 0; e < n; ++e) order[e] = e;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    if (long_first(a) != long_first(b)) return long_first(a);
+    if (group(a) != group(b)) return group(a) < group(b);
+    // the first part longest first
+    return long_first(a) && est[a].longest > est[b].longest;
+  });
+  uint64_t total = 0;
+  for (uint32_t e = 0; e < n; ++e) total += (photons[e] + chunk - 1) / chunk;
+  std::vector<unsigned long long> slots;
+  slots.reserve(total);
+  // round j of a part: chunk j of every event of the part that has one
+  // (events drop out of `live` once their chunks are used up)
+  auto emit_part = [&](size_t lo, size_t hi) {
+    std::vector<uint32_t> live(order.begin() + lo, order.begin() + hi);
+    for (uint64_t j = 0; !live.empty(); ++j) {
+      size_t keep = 0;
+      for (const uint32_t e : live) {
+        const uint64_t chunks = (photons[e] + chunk - 1) / chunk;
+        if (j < chunks) {
+          slots.push_back((static_cast<unsigned long long>(e) << 32) | j);
+          if (j + 1 < chunks) live[keep++] = e;
+        }
+      }
+      live.resize(keep);
+    }
+  };
+  size_t split = 0;
+  while (split < n && long_first(order[split])) ++split;
+  emit_part(0, split);
+  emit_part(split, n);
+  return slots;
+}
+
 template <class F>
 int guarded(F&& f) {
   try {
@@ -1148,21 +1244,10 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     const uint32_t words = 6u + time_bins;
     const size_t count_words = static_cast<size_t>(n_events) * words;
     const size_t zero_u64 = (count_words + 1) / 2;
-    // Queue order: events grouped by branch structure (stable within a group).
-    std::vector<uint32_t> order(n_events);
-    for (uint32_t e = 0; e < n_events; ++e) order[e] = e;
-    auto group = [&](uint32_t e) {
-      return params[e].surface * 4u + (params[e].scat_length != kInf ? 2u : 0u) +
-             (params[e].abs_length != kInf ? 1u : 0u);
-    };
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return group(a) < group(b); });
-
     const auto ta = std::chrono::steady_clock::now();
     bool grew = c.batch_table.ensure(table.size() * sizeof(DevParams));
     grew |= c.batch_offsets.ensure(offsets.size() * sizeof(unsigned long long));
     grew |= c.batch_counts.ensure(zero_u64 * 8);
-    grew |= c.batch_order.ensure(order.size() * sizeof(uint32_t));
     if (grew) {
       tm.alloc_ms += ms_since(ta);
       c.drop_graphs();  // buffers moved: captured graphs are stale
@@ -1187,6 +1272,13 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     }();
     if (s.queue_chunk < 64 || s.queue_chunk % 64 != 0 || s.steps_per_refill == 0)
       throw InvalidArgument("batch: queue_chunk must be a multiple of 64, steps_per_refill >= 1");
+    const std::vector<unsigned long long> slots = batch_slots(
+        params, photons, n_events, s.queue_chunk, static_cast<uint64_t>(s.grid) * s.block);
+    const auto tsl = std::chrono::steady_clock::now();
+    if (c.batch_slots.ensure(std::max<size_t>(1, slots.size()) * sizeof(unsigned long long))) {
+      tm.alloc_ms += ms_since(tsl);
+      c.drop_graphs();
+    }
     unsigned long long* scratch = ensure_scratch(c, 1, tm.alloc_ms);
 
     const auto th = std::chrono::steady_clock::now();
@@ -1195,8 +1287,9 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     HIPCHK(hipMemcpyAsync(c.batch_offsets.ptr, offsets.data(),
                           offsets.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
                           c.stream));
-    HIPCHK(hipMemcpyAsync(c.batch_order.ptr, order.data(), order.size() * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipMemcpyAsync(c.batch_slots.ptr, slots.data(),
+                          slots.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                          c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     tm.h2d_ms = ms_since(th);
 
@@ -1209,14 +1302,11 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     lo.evt.time_scale = static_cast<float>(time_bins) / time_max;  // transport.hpp:171
     lo.evt.table = c.batch_table.as<DevParams>();
     lo.evt.offsets = c.batch_offsets.as<unsigned long long>();
-    lo.evt.order = c.batch_order.as<uint32_t>();
+    lo.evt.slots = c.batch_slots.as<unsigned long long>();
     lo.evt.n_events = n_events;
     const PhiloxKey key = key_of(params[0]);
 
-    uint64_t max_chunks = 0;
-    for (uint32_t e = 0; e < n_events; ++e)
-      max_chunks = std::max<uint64_t>(max_chunks, (photons[e] + s.queue_chunk - 1) / s.queue_chunk);
-    const uint64_t n_slots = max_chunks * n_events;
+    const uint64_t n_slots = slots.size();
     s.slots_per_claim = slots_per_claim(n_slots, static_cast<uint64_t>(s.grid) * s.block / 64u);
     if (method == OPTPHOT_BATCH_WAVEFRONT) {
       const std::string gkey = std::to_string(n_events) + "/" + std::to_string(n_slots) + "/" +
@@ -1496,6 +1586,16 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
       sf.queue_chunk = tf.queue_chunk;
       sf.stream_in_lds = ring ? 1u : 0u;
     }
+    const std::vector<unsigned long long> slots = batch_slots(
+        params, photons, n_sets, sf.queue_chunk, static_cast<uint64_t>(sf.grid) * sf.block);
+    const auto tsl = std::chrono::steady_clock::now();
+    if (c.batch_slots.ensure(std::max<size_t>(1, slots.size()) * sizeof(unsigned long long))) {
+      tm.alloc_ms += ms_since(tsl);
+      c.drop_graphs();
+    }
+    HIPCHK(hipMemcpyAsync(c.batch_slots.ptr, slots.data(),
+                          slots.size() * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                          c.stream));
     LaunchOut lo{};
     lo.events = true;
     lo.evt.counts = c.batch_counts.as<uint32_t>();
@@ -1504,13 +1604,10 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
     lo.evt.time_scale = static_cast<float>(time_bins) / time_max;  // transport.hpp:171
     lo.evt.table = c.batch_table.as<DevParams>();
     lo.evt.offsets = c.batch_offsets.as<unsigned long long>();
+    lo.evt.slots = c.batch_slots.as<unsigned long long>();
     lo.evt.n_events = n_sets;
     const PhiloxKey key = key_of(params[0]);
-    uint64_t max_chunks = 0;
-    for (uint32_t e = 0; e < n_sets; ++e)
-      max_chunks =
-          std::max<uint64_t>(max_chunks, (photons[e] + sf.queue_chunk - 1) / sf.queue_chunk);
-    const uint64_t n_slots = max_chunks * n_sets;
+    const uint64_t n_slots = slots.size();
     sf.slots_per_claim = slots_per_claim(n_slots, static_cast<uint64_t>(sf.grid) * sf.block / 64u);
 
     // adjoint shape and arguments

@@ -53,8 +53,8 @@ struct DevEventTally {
   float time_scale;                 // time_bins / time_max
   const DevParams* table;           // [n_events] per-event parameters (batched kernel)
   const unsigned long long* offsets;  // [n_events + 1] prefix sums of photons per event
-  const uint32_t* order;            // [n_events] queue order: slot c is chunk c / E of
-                                    // event order[c mod E] (nullptr: event c mod E)
+  const unsigned long long* slots;  // [n_slots] batch queue: slot c is chunk
+                                    // (slots[c] & 0xffffffff) of event slots[c] >> 32
   uint32_t n_events;
   uint32_t event;                   // the single event of a plain per-event launch
 };
@@ -190,9 +190,9 @@ void launch_wavefront(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
 
 // Batched events: one persistent launch over all events (event e owns ids
 // first_id + offsets[e] .. first_id + offsets[e+1] - 1), per-lane event
-// parameters from out.evt.table, per-event tallies. The queue has n_slots =
-// n_events * max_e ceil(photons_e / queue_chunk) slots; slot c is chunk
-// c / n_events of event order[c mod n_events] (empty slots are skipped).
+// parameters from out.evt.table, per-event tallies. The queue hands out the
+// n_slots slots of out.evt.slots in order (one per chunk of queue_chunk
+// photons of an event).
 void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
                             const WavefrontShape& shape, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream);

@@ -36,10 +36,20 @@ namespace {
 
 // Batched events and gensteps: a wave claims `per_claim` consecutive queue
 // slots per atomic (slots_per_claim() on the host: at most 16, and few enough
-// that every wave gets at least 4 claims). For events, slot c is chunk c / E of
-// event c mod E, so a claim spans several events; empty slots (events with
-// fewer chunks than the largest) cost the wave a compare instead of a
-// contended global atomic.
+// that every wave gets at least 4 claims). A slot is one chunk of one event
+// or genstep, from a host-built slot table.
+
+// Steps into a history beyond which a batch wave raises its issue priority.
+constexpr uint32_t kLongHistory = 256u;
+
+// Wave issue priority 2 while any lane's `is_long` holds, 0 otherwise (the
+// priority only orders this kernel's waves against each other on a SIMD).
+__device__ __forceinline__ void set_priority_for_long_history(bool is_long) {
+  if (__ballot(is_long) != 0)
+    __builtin_amdgcn_s_setprio(2);
+  else
+    __builtin_amdgcn_s_setprio(0);
+}
 
 // Wave-uniform branch-event counters (kept in SGPRs). Compiled in only for
 // the launches that report them: 16 live 64-bit scalars otherwise push the
@@ -878,9 +888,8 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   LaunchOut loc = out;
   if (kLds) loc.evt.counts = evl;
   const uint32_t lane = lane_id();
-  const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
-  const uint32_t* ord = out.evt.order;
+  const unsigned long long* slots = out.evt.slots;
   // every event carries the same table pointer; read it once, as a uniform
   const float* const rtab = out.evt.table[0].rayleigh;
   WaveCounters<kCount> wc;
@@ -900,6 +909,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   Photon ph;
 
   for (uint32_t it = 0;; ++it) {
+    set_priority_for_long_history(alive && done >= kLongHistory);
     const bool all_dead = __ballot(alive) == 0;
     if (all_dead || it % steps_per_refill == 0) {
       if (pending) {
@@ -929,8 +939,9 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
             s_end = (c + per_claim < n_slots) ? c + per_claim : n_slots;
           }
           const uint64_t c = s_next++;
-          const uint32_t e = ord ? ord[c % E] : static_cast<uint32_t>(c % E);
-          const uint64_t start = (c / E) * chunk;
+          const unsigned long long sl = slots[c];
+          const uint32_t e = static_cast<uint32_t>(sl >> 32);
+          const uint64_t start = static_cast<uint64_t>(static_cast<uint32_t>(sl)) * chunk;
           const uint64_t ne = off[e + 1] - off[e];
           if (start >= ne) continue;
           we = e;
@@ -1247,13 +1258,22 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
 // ---------------------------------------------------------------------------
 // Batched events. Lanes take photons of many events and carry their event's
 // parameters in registers (loaded when the lane is refilled). The queue hands
-// out slots c = 0, 1, ...; slot c is chunk c / E of event order[c mod E], so
-// the waves in flight at any moment work on many different events and the
-// per-event tally atomics spread over many rows instead of queueing on one.
-// The host orders the events by their branch structure (surface model,
-// scattering on, absorption on), so the consecutive slots a wave claims
-// mostly share it: branch bodies that none of the wave's events need (the
-// rotation when no event scatters or reflects diffusely) are then skipped.
+// out slots c = 0, 1, ...; slot c is one chunk of one event (out.evt.slots).
+// The host lays the slots out so that
+//  * the events expected to have the longest histories come first: a
+//    persistent launch lasts at least as long as its longest photon history,
+//    and that history then runs alongside the bulk instead of after it;
+//  * within that order, chunk j of every event precedes chunk j + 1, so the
+//    waves in flight at any moment work on many different events and the
+//    per-event tally atomics spread over many rows instead of queueing on one;
+//  * events with the same branch structure (surface model, scattering on,
+//    absorption on) are adjacent, so the consecutive slots a wave claims
+//    mostly share it: branch bodies that none of the wave's events need (the
+//    rotation when no event scatters or reflects diffusely) are then skipped.
+// A wave that holds a photon more than kLongHistory steps into its history
+// raises its issue priority until it no longer does, so that the few long
+// histories, which bound the launch, are not slowed by the waves they share a
+// SIMD with.
 //
 // With kLds (few events: n_events * words u32 fit in LDS) each block keeps
 // its own copy of all event tallies in LDS and adds it to the global rows
@@ -1273,9 +1293,8 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
   LaunchOut loc = out;
   if (kLds) loc.evt.counts = lds;  // finish() adds into the block's copy
   const uint32_t lane = lane_id();
-  const uint32_t E = out.evt.n_events;
   const unsigned long long* off = out.evt.offsets;
-  const uint32_t* ord = out.evt.order;
+  const unsigned long long* slots = out.evt.slots;
   // every event carries the same table pointer; read it once, as a uniform
   const float* const rtab = out.evt.table[0].rayleigh;
   WaveCounters<kCount> wc;
@@ -1295,6 +1314,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
   Photon ph;
 
   for (;;) {
+    set_priority_for_long_history(alive && done >= kLongHistory);
     const uint64_t dead = __ballot(!alive);
     if (dead != 0 && !q_done) {
       const uint32_t need = popc(dead);
@@ -1318,8 +1338,9 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
           s_end = (c + per_claim < n_slots) ? c + per_claim : n_slots;
         }
         const uint64_t c = s_next++;
-        const uint32_t e = ord ? ord[c % E] : static_cast<uint32_t>(c % E);
-        const uint64_t start = (c / E) * chunk;
+        const unsigned long long sl = slots[c];
+        const uint32_t e = static_cast<uint32_t>(sl >> 32);
+        const uint64_t start = static_cast<uint64_t>(static_cast<uint32_t>(sl)) * chunk;
         const uint64_t ne = off[e + 1] - off[e];
         if (start >= ne) continue;
         we = e;

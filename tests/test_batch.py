@@ -59,6 +59,10 @@ def test_small_batch_matches_oracle_event_by_event(backend, oracle_portable):
         assert np.array_equal(w[e].astype(np.int64), _oracle_row(oracle_portable, b.params[e],
                                                                   b.photons[e], off[e])), e
     assert w[:, :5].sum() == b.total
+    # every event empty: an empty queue
+    b.photons[:] = 0
+    z, _, _ = run_gpu_batch(b, TB, TMAX, method=WAVEFRONT)
+    assert not z.any()
 
 
 @pytest.mark.gpu
