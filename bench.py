@@ -220,15 +220,18 @@ def bench_batched():
     abs ~U[300,3000] mm, scat inf or ~U[100,1000] mm, point source uniform,
     N ~ Poisson(1e4 E), E ~U[0.1,3] MeV; generator seed 2024, run seed 1),
     per-event detected count + 128-bin time histogram 0-50 ns. One persistent
-    graph-captured launch vs one history-kernel launch per event."""
+    graph-captured launch (the autotuned batch shape of this batch class, if
+    tuning.json holds one) vs one history-kernel launch per event."""
     import numpy as np
 
+    from gpu_optical_photons_amd import autotune
     from gpu_optical_photons_amd.batch import PLAIN, WAVEFRONT, make_events, run_gpu_batch
     b = make_events(4096, seed=2024)
-    run_gpu_batch(b, 128, 50.0, method=WAVEFRONT)  # capture
-    w, tw, _ = run_gpu_batch(b, 128, 50.0, method=WAVEFRONT)
+    tu = autotune.batch_choice_for(b, 128)
+    run_gpu_batch(b, 128, 50.0, method=WAVEFRONT, tuning=tu)  # capture
+    w, tw, _ = run_gpu_batch(b, 128, 50.0, method=WAVEFRONT, tuning=tu)
     pl, tp, _ = run_gpu_batch(b, 128, 50.0, method=PLAIN)
-    return {"events": b.n_events, "photons": b.total, "wavefront_ms": tw.kernel_ms,
+    return {"events": b.n_events, "photons": b.total, "tuning": tu, "wavefront_ms": tw.kernel_ms,
             "plain_ms": tp.kernel_ms, "wavefront_photons_per_s": b.total / (tw.kernel_ms * 1e-3),
             "plain_photons_per_s": b.total / (tp.kernel_ms * 1e-3),
             "identical": bool(np.array_equal(w, pl))}
@@ -278,7 +281,7 @@ def bench_calibration():
     return {k: r[k] for k in ("n_sets", "photons_per_set", "iterations", "median_rel_error",
                               "p90_rel_error", "graph_ms", "forward_only_ms", "wall_s",
                               "forward_photons_per_s", "fwd_adj_photons_per_s",
-                              "fwd_adj_over_fwd")} | {
+                              "fwd_adj_over_fwd", "forward_tuning")} | {
         "final_median_loss": r["losses"][-1], "first_median_loss": r["losses"][0]}
 
 

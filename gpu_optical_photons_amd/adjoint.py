@@ -46,10 +46,11 @@ def with_theta(template: SimParams, theta) -> SimParams:
     return template.copy(**{k: float(v) for k, v in zip(THETA, theta)})
 
 
-def histograms(batch: EventBatch, time_bins: int, time_max: float, first_id: int = 0):
+def histograms(batch: EventBatch, time_bins: int, time_max: float, first_id: int = 0,
+               tuning=None):
     """h [E, time_bins + 1] (fractions of each set's photons per arrival-time
     bin, detected photons only) and the RunTiming of the launch."""
-    counts, tm, _ = run_gpu_batch(batch, time_bins, time_max, first_id=first_id)
+    counts, tm, _ = run_gpu_batch(batch, time_bins, time_max, first_id=first_id, tuning=tuning)
     n = np.maximum(batch.photons.astype(np.float64), 1.0)[:, None]
     return counts[:, NUM_FATES:].astype(np.float64) / n, tm
 

@@ -4,7 +4,9 @@ Tunes block size, the refill policy -- steps between refills ("wavefront
 length"), or the successor kernel's emission threshold (lanes lacking a
 successor before a wave emits) --, the queue chunk a wave claims per atomic,
 and persistent blocks per CU, by timing the real kernel on a sample of the
-workload. Results never depend on the choice
+workload; for batches of events (one persistent launch over all events),
+block size, steps between refills, queue chunk and the streams' place
+(registers or LDS rings), by timing the batch itself. Results never depend on the choice
 (integer tallies, per-photon streams; checked by the sweep itself and by the
 GPU tests), only the speed does. Choices persist in tuning.json next to this
 file, keyed by a workload signature (kernel, surface model, source, which bulk
@@ -121,6 +123,78 @@ def sweep(kernel: str, p, c, n: int, blocks=BLOCKS, steps=STEPS, chunks=CHUNKS,
     return entry
 
 
+BATCH_BLOCKS = (256, 512, 1024)
+BATCH_STEPS = (1, 2, 3)
+BATCH_CHUNKS = (512, 1024, 2048)
+
+
+def batch_signature(batch, time_bins: int) -> str:
+    """Workload class of a batch: how many events, how many photons per event,
+    which surface models occur, whether scattering is on, and whether the
+    per-event rows fit the LDS of a block."""
+    E = batch.n_events
+    mean = float(batch.total) / max(E, 1)
+    surfaces = "".join(sorted({str(int(p.surface)) for p in batch.params}))
+    scat = {math.isinf(p.scat_length) for p in batch.params}
+    rows_lds = E * (6 + time_bins) * 4 <= 32 * 1024
+    return (f"batch|events={'few' if E < 64 else 'mid' if E < 1024 else 'many'}"
+            f"|photons={'<1e4' if mean < 1e4 else '<1e5' if mean < 1e5 else '>=1e5'}"
+            f"|surfaces={surfaces}"
+            f"|scat={'off' if scat == {True} else 'on' if scat == {False} else 'mixed'}"
+            f"|rows={'lds' if rows_lds else 'global'}")
+
+
+def batch_choice_for(batch, time_bins: int):
+    """The stored tuning for a batch of this class, or None (library defaults)."""
+    entry = _load().get(batch_signature(batch, time_bins))
+    return dict(entry["tuning"]) if entry else None
+
+
+def sweep_batch(batch, time_bins: int, time_max: float, blocks=BATCH_BLOCKS,
+                steps=BATCH_STEPS, chunks=BATCH_CHUNKS, rings=(0, 1), reps: int = 2,
+                save: bool = True, log=print) -> dict:
+    """Times every batch launch shape on the batch itself (one persistent
+    launch per call); keeps the fastest. Every candidate must give the same
+    per-event tallies word for word."""
+    import numpy as np
+
+    from .batch import run_gpu_batch
+
+    ref = None
+    results = []
+    for b, k, ch, ring in itertools.product(blocks, steps, chunks, rings):
+        tu = {"block_size": b, "steps_per_refill": k, "queue_chunk": ch, "stream_in_lds": ring}
+        try:
+            run_gpu_batch(batch, time_bins, time_max, tuning=tu)  # captures the graph
+            best = math.inf
+            for _ in range(reps):
+                w, tm, _ = run_gpu_batch(batch, time_bins, time_max, tuning=tu)
+                best = min(best, tm.kernel_ms)
+        except ValueError as e:  # a shape this batch does not fit (LDS)
+            log(f"  skipped {tu}: {e}")
+            continue
+        if ref is None:
+            ref = w
+        elif not np.array_equal(ref, w):
+            raise AssertionError(f"batch tallies differ for tuning {tu}: the kernel is not "
+                                 "schedule-independent")
+        results.append((best, tu))
+        log(f"  batch block={b:4d} steps={k} chunk={ch:5d} ring={ring}: {best:8.3f} ms")
+    results.sort(key=lambda r: r[0])
+    best_ms, best_tu = results[0]
+    default = {"block_size": 256, "steps_per_refill": 2, "queue_chunk": 1024, "stream_in_lds": 0}
+    entry = {"tuning": best_tu, "kernel_ms": best_ms, "photons": int(batch.total),
+             "events": int(batch.n_events), "photons_per_s": batch.total / (best_ms * 1e-3),
+             "default_ms": next((ms for ms, tu in results if tu == default), None),
+             "swept": len(results), "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+    if save:
+        cache = _load()
+        cache[batch_signature(batch, time_bins)] = entry
+        with open(CACHE, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return entry
+
+
 def main():
     import sys
     sys.path.insert(0, os.path.dirname(HERE))
@@ -133,6 +207,13 @@ def main():
                        ("plain small", c1, TallyConfig())):
         print(f"sweep: {name}", flush=True)
         e = sweep("wavefront_tally", p, c, 20_000_000, log=lambda s: print(s, flush=True))
+        print(f"best: {e['tuning']} {e['kernel_ms']:.3f} ms", flush=True)
+    from . import calibrate
+    from .batch import make_events
+    for name, batch, tb, tmax in (("batched events", make_events(4096, seed=2024), 128, 50.0),
+                                  ("calibration sets", calibrate.c5_batch(), 128, 100.0)):
+        print(f"sweep: {name}", flush=True)
+        e = sweep_batch(batch, tb, tmax, log=lambda s: print(s, flush=True))
         print(f"best: {e['tuning']} {e['kernel_ms']:.3f} ms", flush=True)
 
 

@@ -125,13 +125,27 @@ def c5_template(seed: int) -> SimParams:
                      max_steps=10000, seed=seed)
 
 
+def c5_batch(n_sets=256, photons=1_000_000, draw_seed=5, seed=1) -> EventBatch:
+    """A forward batch of the calibration: every set at its hidden parameters
+    (the draw of calibrate() with the same draw_seed), `photons` each."""
+    rng = np.random.default_rng(draw_seed)
+    th = np.tile(np.array([2000.0, 300.0, 0.97, 1.5, 1.63]), (n_sets, 1))
+    th[:, list(FIT)] = LO + (HI - LO) * rng.random((n_sets, 3))
+    tpl = c5_template(seed)
+    return EventBatch([adjoint.with_theta(tpl, r) for r in th],
+                      np.full(n_sets, photons, np.uint64))
+
+
 def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=0.08,
               draw_seed=5, seed=1, target_seed=99, target_photons=None, log=None,
               time_forward=True):
     """Fits (abs_length, R, n_det) of every set to its own target histogram.
     Returns a dict with hidden / fitted parameters, errors and timings; with
     time_forward, each iteration also times a forward-only batch at the same
-    parameters (outside the fit) for the forward+adjoint / forward ratio."""
+    parameters (outside the fit) for the forward+adjoint / forward ratio. The
+    forward launches take the autotuned batch shape for this batch class, if
+    any."""
+    from . import autotune
     rng = np.random.default_rng(draw_seed)
     hidden = LO + (HI - LO) * rng.random((n_sets, 3))
     target_photons = target_photons or photons
@@ -153,6 +167,7 @@ def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=
     v = np.zeros_like(x)
     b1, b2, eps = 0.9, 0.999, 1e-8
     tpl = c5_template(seed)
+    tuning = None
     step_ms = fwd_ms = 0.0
     wall0 = time.perf_counter()
     losses = []
@@ -160,11 +175,14 @@ def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=
         th = theta_of(x)
         batch = EventBatch([adjoint.with_theta(tpl, r) for r in th],
                            np.full(n_sets, photons, np.uint64))
+        if it == 1:
+            tuning = autotune.batch_choice_for(batch, bins)
         # one captured graph per iteration: forward, chi-square weights, adjoint
-        _, loss, g, ts = adjoint.run_gpu_chi2_step(batch, T, target_photons, bins, tmax)
+        _, loss, g, ts = adjoint.run_gpu_chi2_step(batch, T, target_photons, bins, tmax,
+                                                   forward_tuning=tuning)
         step_ms += ts.kernel_ms
         if time_forward:  # forward alone at the same parameters, for the cost ratio
-            fwd_ms += adjoint.histograms(batch, bins, tmax)[1].kernel_ms
+            fwd_ms += adjoint.histograms(batch, bins, tmax, tuning=tuning)[1].kernel_ms
         gx = g[:, list(FIT)] * (HI - LO)  # chain rule to normalised coordinates
         m = b1 * m + (1 - b1) * gx
         v = b2 * v + (1 - b2) * gx * gx
@@ -188,4 +206,5 @@ def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=
         "forward_photons_per_s": total / (fwd_ms * 1e-3) if fwd_ms else None,
         "fwd_adj_photons_per_s": total / (step_ms * 1e-3),
         "fwd_adj_over_fwd": step_ms / fwd_ms if fwd_ms else None,
+        "forward_tuning": tuning,
     }

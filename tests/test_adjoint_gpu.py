@@ -102,11 +102,15 @@ def test_gradient_agrees_with_finite_differences(backend):
             assert abs(lr[j] - fd[j]) <= 4.0 * se, name
 
 
-@pytest.mark.parametrize("n_sets", [1, 6])
-def test_chi2_step_graph_equals_separate_calls(backend, n_sets):
+@pytest.mark.parametrize("n_sets,ftu", [
+    (1, None), (6, None),
+    # the calibration sets' autotuned forward shape: LDS rings, long chunks
+    (6, dict(block_size=512, steps_per_refill=2, queue_chunk=2048, stream_in_lds=1))])
+def test_chi2_step_graph_equals_separate_calls(backend, n_sets, ftu):
     """The one-graph calibration step (forward, chi-square weights on the
     device, adjoint) against the same quantities from separate calls, over
-    three replays with new parameters (the graph must read them afresh)."""
+    three replays with new parameters (the graph must read them afresh), with
+    the default and a tuned forward launch shape."""
     from gpu_optical_photons_amd import adjoint
     from gpu_optical_photons_amd.batch import EventBatch
     from gpu_optical_photons_amd.calibrate import c5_template
@@ -122,7 +126,8 @@ def test_chi2_step_graph_equals_separate_calls(backend, n_sets):
         batch = EventBatch([adjoint.with_theta(tpl, r) for r in th],
                            np.full(n_sets, n, np.uint64))
         print(f"[chi2 step] sets {n_sets} replay {rep}", flush=True)
-        hist, loss, grad, _ = adjoint.run_gpu_chi2_step(batch, T, nt, bins, tmax)
+        hist, loss, grad, _ = adjoint.run_gpu_chi2_step(batch, T, nt, bins, tmax,
+                                                        forward_tuning=ftu)
         h, _ = adjoint.histograms(batch, bins, tmax)
         assert np.array_equal(hist, h)
         np.testing.assert_allclose(loss, adjoint.chi2_loss(h, T, n, nt), rtol=1e-12)

@@ -38,6 +38,20 @@ def test_batch_rejects_mixed_seeds_without_gpu():
         run_gpu_batch(EventBatch([SimParams()], np.array([1], np.uint64)), time_bins=0)
 
 
+def test_batch_signature_classes():
+    from gpu_optical_photons_amd import autotune, calibrate
+    from gpu_optical_photons_amd.batch import make_events
+    c3 = autotune.batch_signature(make_events(4096, seed=2024), 128)
+    assert c3 == "batch|events=many|photons=<1e5|surfaces=0123|scat=mixed|rows=global"
+    c5 = autotune.batch_signature(calibrate.c5_batch(n_sets=256, photons=10), 128)
+    assert c5 == "batch|events=mid|photons=<1e4|surfaces=3|scat=on|rows=global"
+    few = autotune.batch_signature(make_events(8, seed=12, photons_per_mev=3000), 128)
+    assert few.endswith("|rows=lds") and "|events=few|" in few
+    tu = autotune.batch_choice_for(make_events(4096, seed=2024), 128)
+    assert tu is None or set(tu) == {"block_size", "steps_per_refill", "queue_chunk",
+                                      "stream_in_lds"}
+
+
 @pytest.mark.gpu
 def test_small_batch_matches_oracle_event_by_event(backend, oracle_portable):
     from gpu_optical_photons_amd.batch import PLAIN, WAVEFRONT, make_events, run_gpu_batch
