@@ -307,6 +307,9 @@ def main():
     ap.add_argument("--sorted-only", action="store_true",
                     help="only the sorting-by-interaction-type measurements on the "
                          "long-histories workload")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched-events measurement (4096 events, autotuned shape, "
+                         "against one history launch per event)")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the library's VALU calibration kernel (all lanes, then odd "
                          "lanes only): a known instruction count for reading rocprofv3 VALU "
@@ -348,6 +351,9 @@ def main():
                         "lane_fma_per_s": waves * 8 * iters * (32 if half else 64)
                         / (ms.value * 1e-3)}
         print(json.dumps(out))
+        return
+    if args.batched_only:
+        print(json.dumps({"batched_events": bench_batched()}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
