@@ -47,7 +47,8 @@ def c4_problem(n: int, n_target: int | None = None, seed: int = 1, target_seed: 
 
 
 def loss_and_grad(template, theta, target, n, n_target, first_id=0):
-    """C4 loss at theta and its adjoint gradient (same photons)."""
+    """The single-geometry chi-square loss at theta and its adjoint gradient (same
+    photons)."""
     p = adjoint.with_theta(template, theta)
     b = EventBatch([p], np.array([n], np.uint64))
     h, tf = adjoint.histograms(b, C4_BINS, C4_TMAX, first_id)

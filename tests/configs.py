@@ -1,8 +1,10 @@
 """Named SimParams used across the tests.
 
 The reference's configs/*.cfg (solid_angle, absorption, default,
-wrapped_lambertian) and the BASELINE workloads: C1 plain small, C2 long
-histories. Values restated from configs/*.cfg and params.hpp:42-62.
+wrapped_lambertian) and the benchmark workloads: C1 plain small (50 mm
+polished cube, 10^6 photons) and C2 long histories (100 x 100 x 50 mm
+Lambertian box, 10^8 photons). Values restated from configs/*.cfg and
+params.hpp:42-62.
 """
 import math
 
