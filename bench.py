@@ -105,6 +105,15 @@ def bench_plain_small():
                                   dev_w[f].cpu().numpy().view(np.uint8))
                for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"))
     steps = int(rec.steps().sum())
+    # the same pipeline at 10x the photons, into fresh arrays (pages first
+    # touched by the copies) and into arrays a caller reuses
+    n10 = 10 * n
+    backend.run_gpu_records(p, n10)
+    fresh10 = min(backend.run_gpu_records(p, n10)[1].total_ms for _ in range(2))
+    out10 = backend.PhotonRecords.empty(n10)
+    for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"):
+        getattr(out10, f).fill(0)
+    reused10 = min(backend.run_gpu_records(p, n10, out=out10)[1].total_ms for _ in range(2))
     return {"photons": n, "history_kernel_ms": tm.kernel_ms,
             "history_photons_per_s": n / (tm.kernel_ms * 1e-3),
             "history_photon_steps_per_s": steps / (tm.kernel_ms * 1e-3),
@@ -114,7 +123,11 @@ def bench_plain_small():
             "records_to_host_total_ms": total_ms, "records_to_host_d2h_ms": d2h_ms,
             "records_to_host_photons_per_s": n / (total_ms * 1e-3),
             "records_to_host_GBps": n * 29 / (total_ms * 1e-3) / 1e9,
-            "records_identical": bool(same), "photon_steps": steps}
+            "records_identical": bool(same), "photon_steps": steps,
+            "records_to_host_10x": {
+                "photons": n10, "fresh_arrays_ms": fresh10, "reused_arrays_ms": reused10,
+                "fresh_GBps": n10 * 29 / (fresh10 * 1e-3) / 1e9,
+                "reused_GBps": n10 * 29 / (reused10 * 1e-3) / 1e9}}
 
 
 def bench_gensteps():
