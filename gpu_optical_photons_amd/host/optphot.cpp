@@ -114,7 +114,7 @@ int main(int argc, char** argv) {
       Tally tally;
       RunTiming tm;
       if (wave) {
-        WavefrontTuning tu{o.block_size, steps_per_launch, 0, 0, 0, 0};
+        WavefrontTuning tu{o.block_size, steps_per_launch, 0, 0, 0, 0, 0};
         tm = run_gpu_tally_wavefront(p, o.tally, o.n_photons, tally, 0, &tu);
       } else {
         tm = run_gpu_tally(p, o.tally, o.n_photons, tally, o.block_size);

@@ -1,19 +1,23 @@
 // Unit tests of the C++ host layer: configuration parsing and validation,
 // config files with command-line overrides, the .npy writer (after the
 // reference's tests/test_config.cpp), the backend.hpp value types and error
-// mapping. With the argument "gpu" it also runs the GPU entry points through
-// backend.hpp and checks the history and wavefront kernels against each other.
+// mapping, and the slot table of batched launches. With the argument "gpu" it
+// also runs the GPU entry points through backend.hpp and checks the history
+// and wavefront kernels against each other.
 // Exit status 0 when every check passes.
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <iterator>
+#include <limits>
+#include <vector>
 #include <stdexcept>
 #include <string>
 
 #include "optphot_hip/backend.hpp"
 #include "optphot_hip/config.hpp"
+#include "../csrc/batch_queue.hpp"
 
 using namespace optphot;
 
@@ -241,6 +245,85 @@ void test_gpu() {
   CHECK(same);
 }
 
+// The batch queue's slot table: every photon of every event in exactly one
+// slot, the events expected to have long histories first (and only when such
+// a history is long against the launch), and chunk j of the events of a part
+// before chunk j + 1.
+void test_batch_slots() {
+  optphot_sim_params base{};
+  base.hx = base.hy = base.hz = 25.0f;
+  base.n_scint = 1.63f;
+  base.n_det = 1.5f;
+  base.n_out = 1.0f;
+  base.abs_length = 1000.0f;
+  base.scat_length = std::numeric_limits<float>::infinity();
+  base.reflectivity = 0.9f;
+  base.max_steps = 10000;
+  base.seed = 1;
+  std::vector<optphot_sim_params> ev;
+  std::vector<uint64_t> ph;
+  uint64_t state = 12345;
+  auto next = [&] { state = state * 6364136223846793005ull + 1442695040888963407ull;
+                    return state >> 33; };
+  for (int e = 0; e < 300; ++e) {
+    optphot_sim_params q = base;
+    q.surface = static_cast<uint32_t>(next() % 4);
+    q.hx = 1.0f + static_cast<float>(next() % 50);
+    q.abs_length = (next() % 5 == 0) ? std::numeric_limits<float>::infinity()
+                                     : 100.0f + static_cast<float>(next() % 3000);
+    ev.push_back(q);
+    ph.push_back(next() % 7 == 0 ? 0 : next() % 5000);
+  }
+  // one heavy event: polished, tiny box, no absorption
+  ev[17].surface = OPTPHOT_SURFACE_POLISHED;
+  ev[17].hx = ev[17].hy = ev[17].hz = 1.0f;
+  ev[17].abs_length = std::numeric_limits<float>::infinity();
+  ph[17] = 3000;
+  for (const uint32_t chunk : {64u, 512u, 1024u}) {
+    for (const uint64_t lanes : {uint64_t{64}, uint64_t{4096}, uint64_t{393216}}) {
+      const auto slots = ophip::batch_slots(ev.data(), ph.data(), 300, chunk, lanes);
+      std::vector<std::vector<int>> seen(300);
+      for (int e = 0; e < 300; ++e) seen[e].assign(ph[e], 0);
+      bool ok = true;
+      for (const auto sl : slots) {
+        const uint32_t e = static_cast<uint32_t>(sl >> 32);
+        const uint64_t start = static_cast<uint32_t>(sl) * uint64_t{chunk};
+        ok &= e < 300 && start < ph[e];
+        if (!ok) break;
+        for (uint64_t p = start; p < std::min<uint64_t>(start + chunk, ph[e]); ++p) ++seen[e][p];
+      }
+      for (int e = 0; ok && e < 300; ++e)
+        for (const int c : seen[e]) ok &= c == 1;
+      CHECK(ok);
+      // with few lanes every history is short against the launch: the plain
+      // layout, chunk 0 of all non-empty events first; with many lanes all of
+      // the heavy event comes before any black-walled event
+      if (lanes == 393216) {
+        size_t last17 = 0, first_black = slots.size();
+        for (size_t i = 0; i < slots.size(); ++i) {
+          const uint32_t e = static_cast<uint32_t>(slots[i] >> 32);
+          if (e == 17u) last17 = i;
+          if (ev[e].surface == OPTPHOT_SURFACE_BLACK) first_black = std::min(first_black, i);
+        }
+        CHECK(last17 < first_black);
+      }
+      if (lanes == 64) {
+        size_t nonempty = 0;
+        for (int e = 0; e < 300; ++e) nonempty += ph[e] > 0;
+        bool first_round = true;
+        for (size_t i = 0; i < nonempty; ++i) first_round &= static_cast<uint32_t>(slots[i]) == 0;
+        CHECK(first_round);
+      }
+    }
+  }
+  const auto h = ophip::history_estimate(ev[17], ph[17]);
+  CHECK(h.longest == 10000.0);  // capped at max_steps
+  optphot_sim_params black = base;
+  black.surface = OPTPHOT_SURFACE_BLACK;
+  CHECK(ophip::history_estimate(black, 1000).mean == 1.0);
+  CHECK(ophip::batch_slots(ev.data(), std::vector<uint64_t>(300, 0).data(), 300, 64, 64).empty());
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -252,6 +335,7 @@ int main(int argc, char** argv) {
     test_meta_json();
     test_value_types();
     test_errors();
+    test_batch_slots();
     if (gpu) test_gpu();
   } catch (const std::exception& e) {
     std::printf("FAILED: unexpected exception: %s\n", e.what());
