@@ -1297,7 +1297,19 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
                               " has an end point outside the box");
       if (!(g.t0 >= 0.0f) || !(g.t1 >= 0.0f) || std::isinf(g.t0) || std::isinf(g.t1))
         throw InvalidArgument("gensteps: genstep times must be finite and >= 0");
-      dev[k] = DevGenstep{g.x0, g.y0, g.z0, g.t0, g.x1, g.y1, g.z1, g.t1, g.n_photons, 0u, total};
+      if (!(g.cone_cos >= 0.0f && g.cone_cos <= 1.0f))
+        throw InvalidArgument("gensteps: genstep " + std::to_string(k) +
+                              " has cone_cos outside [0, 1]");
+      if (g.cone_cos > 0.0f) {
+        // the axis (x1 - x0) / |x1 - x0| of a Cherenkov cone needs a squared
+        // length that is a normal float (the device computes it the same way)
+        const float dx = g.x1 - g.x0, dy = g.y1 - g.y0, dz = g.z1 - g.z0;
+        if (!(dx * dx + dy * dy + dz * dz >= 0x1p-100f))
+          throw InvalidArgument("gensteps: Cherenkov genstep " + std::to_string(k) +
+                                " needs a segment with a direction (|x1 - x0|^2 >= 2^-100)");
+      }
+      dev[k] = DevGenstep{g.x0, g.y0, g.z0, g.t0, g.x1, g.y1, g.z1, g.t1, g.n_photons, g.cone_cos,
+                          total};
       for (uint64_t c0 = 0; c0 < g.n_photons; c0 += tu.queue_chunk)
         slots.push_back(make_uint2(k, static_cast<uint32_t>(c0 / tu.queue_chunk)));
       total += g.n_photons;

@@ -536,7 +536,8 @@ __device__ __forceinline__ void emit(const DevParams& P, RNG& rng, Photon& ph) {
 struct DevGenstep {
   float x0, y0, z0, t0;
   float x1, y1, z1, t1;
-  uint32_t n_photons, pad;
+  uint32_t n_photons;
+  float cone_cos;  // 0: scintillation; (0, 1]: Cherenkov cone
   unsigned long long offset;
 };
 
@@ -546,10 +547,28 @@ struct DevGenstep {
 // phi, emission delay; position x0 + u (x1 - x0) per axis, time
 // t0 + u (t1 - t0) + delay. A degenerate genstep with t0 = 0 draws and computes
 // exactly what the point source at (x0, y0, z0) does (transport.hpp:55-67).
+// Cherenkov gensteps (cone_cos > 0; the host checks that the segment has a
+// direction): draws u along the segment, then phi; position and time as
+// above, no delay, and the direction rotate_about(axis, cone_cos, phi) with
+// axis = (x1 - x0) / |x1 - x0| (IEEE square root and division).
 // Needs 4 words in the stream.
 template <class RNG>
 __device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenstep& G, RNG& rng,
                                              Photon& ph) {
+  if (G.cone_cos > 0.0f) {
+    const float u = to_unit(rng.template peek<0>());
+    const float up = to_unit(rng.template peek<1>());
+    rng.pos += 2u;
+    const float dx = G.x1 - G.x0, dy = G.y1 - G.y0, dz = G.z1 - G.z0;
+    ph.p = V3{G.x0 + u * dx, G.y0 + u * dy, G.z0 + u * dz};
+    const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    ph.d = rotate_about(V3{inv * dx, inv * dy, inv * dz}, G.cone_cos, up);
+    ph.t = G.t0 + u * (G.t1 - G.t0);
+    ph.path = 0.0f;
+    ph.nb = 0u;
+    ph.ns = 0u;
+    return;
+  }
   ph.p = V3{G.x0, G.y0, G.z0};
   float t0 = G.t0;
   if (G.x0 != G.x1 || G.y0 != G.y1 || G.z0 != G.z1 || G.t0 != G.t1) {

@@ -274,22 +274,29 @@ int optphot_run_gpu_adjoint(const optphot_sim_params* params, const uint64_t* ph
                             float time_max, const double* weights, double* grad,
                             const optphot_wavefront_tuning* tuning, optphot_run_timing* timing);
 
-/* A scintillation step summary ("genstep"): photons are emitted uniformly
- * along the segment (x0,y0,z0) at t0 -> (x1,y1,z1) at t1, isotropically,
- * each with the run's Exp(tau) emission delay added. Both end points must lie
- * in the box. 40 bytes. */
+/* A step summary ("genstep"): photons are emitted uniformly along the
+ * segment (x0,y0,z0) at t0 -> (x1,y1,z1) at t1. Both end points must lie in
+ * the box. 40 bytes.
+ *  - cone_cos == 0: scintillation. Isotropic directions, each photon with
+ *    the run's Exp(tau) emission delay added.
+ *  - cone_cos in (0, 1]: Cherenkov light of a charged particle moving along
+ *    the segment. Directions on the cone of half-angle acos(cone_cos) about
+ *    the segment direction (cone_cos = 1 / (beta n) for speed beta c in
+ *    index n), uniform in azimuth, no delay. The segment must not be
+ *    degenerate. */
 typedef struct optphot_genstep {
   float x0, y0, z0, t0;
   float x1, y1, z1, t1;
   uint32_t n_photons;
-  uint32_t reserved; /* 0 */
+  float cone_cos;
 } optphot_genstep;
 
 /* Photons generated on the device from gensteps and transported in the box
  * of `p` (its source fields are unused). Photon k of genstep g has global
  * index i = n_0 + ... + n_{g-1} + k and Philox id first_id + i. Draws per
- * photon: [u along the segment, only if the segment or time span is
- * non-degenerate], cos, phi, emission delay, then the transport steps. A
+ * photon: scintillation [u along the segment, only if the segment or time
+ * span is non-degenerate], cos, phi, emission delay; Cherenkov u along the
+ * segment, phi; then the transport steps. A
  * degenerate genstep with t0 = 0 is the point source at (x0, y0, z0), photon
  * for photon. Writes the tally (as optphot_run_gpu_tally) and, when `records`
  * is non-NULL, records of all photons at their global index (host arrays of

@@ -575,10 +575,26 @@ static int step_photon_adj(const oracle_params* p, stream_t* rng, pstate* s, int
    segment, only if the segment or its time span is non-degenerate], cos, phi,
    emission delay. A degenerate genstep with t0 = 0 therefore draws and
    computes exactly what the reference's point source at (x0, y0, z0) does
-   (transport.hpp:55-67): same words, and t = 0 + delay = delay. */
+   (transport.hpp:55-67): same words, and t = 0 + delay = delay.
+   Cherenkov (cone_cos > 0): u along the segment, phi; the direction on the
+   cone of cone_cos about the segment direction (rotate_about, vec3.hpp:35-53,
+   as for Rayleigh scattering), no delay. */
 static pstate start_genstep_photon(const oracle_params* p, const oracle_genstep* g,
                                    stream_t* rng) {
   pstate s;
+  if (g->cone_cos > 0.0f) {
+    const float u = stream_uniform(rng);
+    const float up = stream_uniform(rng);
+    const float dx = g->x1 - g->x0, dy = g->y1 - g->y0, dz = g->z1 - g->z0;
+    s.pos = v3_make(g->x0 + u * dx, g->y0 + u * dy, g->z0 + u * dz);
+    const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    s.dir = rotate_about(v3_make(inv * dx, inv * dy, inv * dz), g->cone_cos, up);
+    s.t = g->t0 + u * (g->t1 - g->t0);
+    s.path = 0.0f;
+    s.nb = 0u;
+    s.ns = 0u;
+    return s;
+  }
   s.pos = v3_make(g->x0, g->y0, g->z0);
   float t0 = g->t0;
   if (g->x0 != g->x1 || g->y0 != g->y1 || g->z0 != g->z1 || g->t0 != g->t1) {

@@ -33,14 +33,15 @@ typedef struct {
   uint32_t xy_bins;
 } oracle_tally_config;
 
-/* A scintillation step summary: photons are emitted uniformly along the
- * segment (x0,y0,z0,t0) -> (x1,y1,z1,t1), isotropically, with an Exp(tau)
- * emission delay. 40 bytes. */
+/* A step summary: photons are emitted uniformly along the segment
+ * (x0,y0,z0,t0) -> (x1,y1,z1,t1). cone_cos == 0: scintillation, isotropic
+ * with an Exp(tau) emission delay; cone_cos in (0, 1]: Cherenkov, on the cone
+ * of that cosine about the segment direction, no delay. 40 bytes. */
 typedef struct {
   float x0, y0, z0, t0;
   float x1, y1, z1, t1;
   uint32_t n_photons;
-  uint32_t reserved;
+  float cone_cos;
 } oracle_genstep;
 
 typedef struct {

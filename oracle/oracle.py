@@ -143,7 +143,7 @@ class Oracle(_Lib):
     def gensteps(self, params, gensteps, first_id=0, time_bins=100, time_max=50.0, xy_bins=50,
                  records=False, threads=0):
         """Tally (dict) and, with records=True, records of photons from gensteps
-        (structured array with fields x0 y0 z0 t0 x1 y1 z1 t1 n_photons reserved)."""
+        (structured array with fields x0 y0 z0 t0 x1 y1 z1 t1 n_photons cone_cos)."""
         g = np.ascontiguousarray(gensteps)
         assert g.dtype.itemsize == 40
         total = int(g["n_photons"].sum())

@@ -15,6 +15,8 @@ How the emission is pinned:
 * GPU: the kernel equals the oracle bit for bit (tally and records), for any
   launch shape.
 """
+import math
+
 import numpy as np
 import pytest
 from scipy import stats
@@ -95,13 +97,65 @@ def test_gensteps_outside_the_box_rejected_before_gpu():
     assert backend is not None
 
 
+def test_cherenkov_cone_geometry(oracle_portable):
+    """Cherenkov gensteps against the geometry of the cone. A short segment
+    on the z axis, no absorption or scattering, black side walls and an
+    index-matched readout face: every photon flies straight to z = hz and is
+    detected there. Its radial offset over its path is then sin theta of the
+    cone, its azimuth is uniform, and hz minus the path times cos theta gives
+    the emission point, uniform along the segment, at the time the segment
+    assigns it."""
+    cone = 0.95
+    p = SimParams(hx=50.0, hy=50.0, hz=50.0, n_scint=1.6, n_det=1.6, abs_length=math.inf,
+                  scat_length=math.inf, surface=1, tau=4.0, seed=21)
+    g = np.zeros(1, GENSTEP_DTYPE)
+    g["x0"], g["y0"], g["z0"], g["t0"] = 0.0, 0.0, -40.0, 1.0
+    g["x1"], g["y1"], g["z1"], g["t1"] = 0.0, 0.0, -38.0, 1.2
+    g["cone_cos"] = cone
+    n = 50_000
+    g["n_photons"] = n
+    _, r = oracle_portable.gensteps(p, g, records=True)
+    assert np.all(r["fate"] == 0) and np.all(r["n_boundary"] == 1)
+    x, y, path = (r[k].astype(np.float64) for k in ("x", "y", "path"))
+    sin_t = math.sqrt(1.0 - cone * cone)
+    assert np.abs(np.hypot(x, y) / path - sin_t).max() < 1e-5
+    assert stats.kstest((np.arctan2(y, x) / (2 * np.pi)) % 1.0, "uniform").pvalue > 1e-3
+    z_emit = 50.0 - path * cone
+    u = (z_emit + 40.0) / 2.0
+    assert u.min() > -1e-4 and u.max() < 1.0 + 1e-4
+    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    t_emit = r["t"].astype(np.float64) - path * 1.6 / 299.792458
+    assert np.abs(t_emit - (1.0 + 0.2 * u)).max() < 1e-4  # no emission delay
+
+
+def test_cherenkov_genstep_checks_before_gpu():
+    from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = SimParams(hx=10.0, hy=10.0, hz=10.0)
+    g = np.zeros(1, GENSTEP_DTYPE)
+    g["z1"] = 1.0
+    g["n_photons"] = 10
+    g["cone_cos"] = 1.5
+    with pytest.raises(ValueError, match="cone_cos"):
+        run_gpu_gensteps(p, TallyConfig(), g)
+    g["cone_cos"] = float("nan")
+    with pytest.raises(ValueError, match="cone_cos"):
+        run_gpu_gensteps(p, TallyConfig(), g)
+    g["cone_cos"] = 0.7
+    g["z1"] = 0.0  # no direction
+    with pytest.raises(ValueError, match="direction"):
+        run_gpu_gensteps(p, TallyConfig(), g)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["c2_long", "every_process", "c1_plain"])
 def test_gpu_gensteps_equal_oracle(backend, oracle_portable, name):
     from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
     from gpu_optical_photons_amd.params import TallyConfig
     p = PARITY[name]
-    g = make_gensteps(400, p, photons_per_mm=200.0, seed=9)
+    # a third Cherenkov gensteps (cone about the segment, no delay)
+    g = make_gensteps(400, p, photons_per_mm=200.0, seed=9, cherenkov_fraction=0.3)
+    assert (g["cone_cos"] > 0).sum() > 50 and g[11]["cone_cos"] == 0.0
     g["n_photons"][7] = 0
     g[11]["x1"], g[11]["y1"], g[11]["z1"], g[11]["t1"] = g[11]["x0"], g[11]["y0"], g[11]["z0"], g[11]["t0"]
     c = TallyConfig(256, 100.0, 64)
