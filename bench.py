@@ -133,7 +133,8 @@ def bench_plain_small():
 def bench_gensteps():
     """Photons generated on the device from 100 000 gensteps (synthetic track
     segments <= 10 mm in the long-histories box, 200 photons/mm; generator seed
-    1), tally 256 x 100 ns x 64^2: one persistent launch."""
+    1), tally 256 x 100 ns x 64^2: one persistent launch; then the same with
+    a third of the gensteps Cherenkov."""
     from gpu_optical_photons_amd.gensteps import make_gensteps, run_gpu_gensteps
     from gpu_optical_photons_amd.params import TallyConfig
     p = c2_params()
@@ -155,6 +156,16 @@ def bench_gensteps():
     out["kernel"] = best
     out["kernel_ms"] = out[best]["kernel_ms"]
     out["photons_per_s"] = out[best]["photons_per_s"]
+    # the same segments with a third of them Cherenkov gensteps (photons on the
+    # cone about the segment, beta ~U[1/n, 1]), on the faster kernel
+    gc = make_gensteps(100_000, p, photons_per_mm=200.0, seed=1, cherenkov_fraction=1.0 / 3.0)
+    nc = int(gc["n_photons"].sum())
+    tu = (dict(block_size=512, steps_per_refill=2, queue_chunk=256, stream_in_lds=1)
+          if best == "lds_ring" else dict(block_size=512, steps_per_refill=2, queue_chunk=256))
+    run_gpu_gensteps(p, c, gc, tuning=tu)
+    ms = min(run_gpu_gensteps(p, c, gc, tuning=tu)[2].kernel_ms for _ in range(3))
+    out["with_cherenkov"] = {"cherenkov_gensteps": int((gc["cone_cos"] > 0).sum()),
+                             "photons": nc, "kernel_ms": ms, "photons_per_s": nc / (ms * 1e-3)}
     return out
 
 
