@@ -128,6 +128,36 @@ def test_cherenkov_cone_geometry(oracle_portable):
     assert np.abs(t_emit - (1.0 + 0.2 * u)).max() < 1e-4  # no emission delay
 
 
+@pytest.mark.gpu
+def test_gpu_cherenkov_edge_cases_equal_oracle(backend, oracle_portable):
+    """Cherenkov gensteps at the edges of their range, GPU against the oracle
+    bit for bit: cone_cos = 1 (photons along the segment), a cone of nearly
+    90 degrees, the smallest cone_cos, a segment along each axis and one a
+    few 1e-15 mm long, and scintillation gensteps in between."""
+    from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = PARITY["every_process"]
+    cases = [((0, 0, -5), (0, 0, 5), 1.0), ((1, 2, 3), (4, -2, 0), 1e-3),
+             ((0, 0, 0), (1, 1, 1), float(np.float32(1e-38))), ((-3, 0, 0), (3, 0, 0), 0.6),
+             ((0, -3, 0), (0, 3, 0), 0.9), ((1e-15, 0, 0), (3e-15, 0, 0), 0.7),
+             ((0, 0, 0), (2, 0, 1), 0.0), ((2, 2, 2), (2, 2, 2), 0.0)]
+    g = np.zeros(len(cases), GENSTEP_DTYPE)
+    for k, (a, b, cc) in enumerate(cases):
+        g[k]["x0"], g[k]["y0"], g[k]["z0"] = a
+        g[k]["x1"], g[k]["y1"], g[k]["z1"] = b
+        g[k]["t0"], g[k]["t1"] = 0.5 * k, 0.5 * k + 0.1
+        g[k]["cone_cos"] = cc
+        g[k]["n_photons"] = 3000
+    c = TallyConfig(128, 60.0, 32)
+    ref_t, ref_r = oracle_portable.gensteps(p, g, first_id=11, time_bins=128, time_max=60.0,
+                                            xy_bins=32, records=True)
+    t, r, _, _ = run_gpu_gensteps(p, c, g, first_id=11, records=True)
+    for f in FIELDS:
+        assert np.array_equal(getattr(r, f).view(np.uint8), ref_r[f].view(np.uint8)), f
+    assert np.array_equal(t.fate_counts, ref_t["fate_counts"])
+    assert np.array_equal(t.xy_hist, ref_t["xy_hist"])
+
+
 def test_cherenkov_genstep_checks_before_gpu():
     from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
     from gpu_optical_photons_amd.params import TallyConfig
