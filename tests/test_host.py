@@ -1,10 +1,10 @@
 """The C++ host layer (include/optphot_hip/{params,backend,config}.hpp): the
 reference's configuration keys, .npy + meta.json output and backend.hpp API,
-exercised through the compiled unit-test program and the optphot driver.
+exercised through the compiled unit-test program and the optphot program.
 
 CPU: parser / writer / value-type / argument-error checks (the reference's
 tests/test_config.cpp cases plus error paths). GPU: the backend.hpp functions
-through the C-ABI, and the driver's output files compared with the oracle
+through the C-ABI, and the optphot program's output files compared with the oracle
 photon by photon and word by word.
 """
 import json
