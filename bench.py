@@ -134,7 +134,8 @@ def bench_gensteps():
     """Photons generated on the device from 100 000 gensteps (synthetic track
     segments <= 10 mm in the long-histories box, 200 photons/mm; generator seed
     1), tally 256 x 100 ns x 64^2: one persistent launch; then the same with
-    a third of the gensteps Cherenkov."""
+    a third of the gensteps Cherenkov, and the records of the first 10 000
+    gensteps copied into host arrays."""
     from gpu_optical_photons_amd.gensteps import make_gensteps, run_gpu_gensteps
     from gpu_optical_photons_amd.params import TallyConfig
     p = c2_params()
@@ -166,6 +167,15 @@ def bench_gensteps():
     ms = min(run_gpu_gensteps(p, c, gc, tuning=tu)[2].kernel_ms for _ in range(3))
     out["with_cherenkov"] = {"cherenkov_gensteps": int((gc["cone_cos"] > 0).sum()),
                              "photons": nc, "kernel_ms": ms, "photons_per_s": nc / (ms * 1e-3)}
+    # records of the first 10 000 gensteps' photons into fresh host arrays
+    # (29 B per photon, staged through pinned memory)
+    gr = g[:10_000]
+    nr = int(gr["n_photons"].sum())
+    tms = [run_gpu_gensteps(p, c, gr, records=True)[2] for _ in range(3)]
+    tm = min(tms, key=lambda t: t.total_ms)
+    out["records_to_host"] = {"gensteps": len(gr), "photons": nr, "total_ms": tm.total_ms,
+                              "kernel_ms": tm.kernel_ms, "d2h_ms": tm.d2h_ms,
+                              "d2h_GBps": nr * 29 / (tm.d2h_ms * 1e6)}
     return out
 
 

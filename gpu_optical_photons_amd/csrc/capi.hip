@@ -536,6 +536,84 @@ DevRecords offset_records(const DevRecords& r, uint64_t k) {
                     r.path + k};
 }
 
+// A staged piece holds the 8 record fields of m photons, field a at byte
+// offset stride x (bytes per photon of fields 0 .. a-1). Copies them into the
+// caller's arrays at photon `off`, on several threads for large pieces.
+void scatter_slot(Context& c, const char* src, uint64_t stride, uint64_t off, uint64_t m,
+                  const optphot_records* host) {
+  char* dst[8] = {reinterpret_cast<char*>(host->fate + off),
+                  reinterpret_cast<char*>(host->n_boundary + off),
+                  reinterpret_cast<char*>(host->n_scatter + off),
+                  reinterpret_cast<char*>(host->t + off),
+                  reinterpret_cast<char*>(host->x + off),
+                  reinterpret_cast<char*>(host->y + off),
+                  reinterpret_cast<char*>(host->z + off),
+                  reinterpret_cast<char*>(host->path + off)};
+  const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
+  std::vector<std::function<void()>> tasks;
+  const size_t split = m >= (uint64_t{1} << 18) ? c.pool->size() + 1 : 1;
+  for (int a = 0; a < 8; ++a) {
+    const size_t bytes = m * sz[a];
+    const size_t part = (bytes + split - 1) / split;
+    for (size_t b = 0; b < bytes; b += part) {
+      const size_t len = std::min(part, bytes - b);
+      char* d = dst[a] + b;
+      const char* from = src + b;
+      tasks.emplace_back([d, from, len] { std::memcpy(d, from, len); });
+    }
+    src += stride * sz[a];
+  }
+  c.pool->run(tasks);
+}
+
+// Photons per staged copy piece of a run of `cap` photons per device buffer:
+// about an eighth of it, at least kPieceMin, at most one staging slot.
+uint64_t stage_piece(uint64_t cap) {
+  const uint64_t piece = std::max<uint64_t>(kPieceMin, (cap + 7) / 8);
+  return std::min<uint64_t>({(piece + 63) / 64 * 64, kStagePhotonsMax, cap});
+}
+
+// Copies n finished records from the device buffers `buf` into the caller's
+// host arrays: once `ready` (recorded on c.stream) has fired, piece k goes to
+// pinned slot k % kStageSlots on c.copy_stream, and the host scatters each
+// slot into the caller's arrays while later pieces are in flight. Returns
+// when every piece is in the caller's arrays.
+void stage_records_to_host(Context& c, const DevRecords& buf, uint64_t n,
+                           const optphot_records* host, hipEvent_t ready, double& alloc_ms) {
+  if (n == 0) return;
+  const uint64_t piece = stage_piece(n);
+  const auto ta = std::chrono::steady_clock::now();
+  bool grew = false;
+  for (auto& st : c.stage) grew |= st.ensure(piece * kRecordBytes);
+  if (grew) alloc_ms += ms_since(ta);
+  const size_t pieces = static_cast<size_t>((n + piece - 1) / piece);
+  c.ensure_piece_events(pieces);
+  hipEvent_t* cev = c.piece_ev.data();
+  const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
+  HIPCHK(hipStreamWaitEvent(c.copy_stream, ready, 0));
+  size_t drained = 0;
+  auto drain = [&](size_t k) {
+    HIPCHK(hipEventSynchronize(cev[k]));
+    const uint64_t off = k * piece;
+    scatter_slot(c, static_cast<const char*>(c.stage[k % Context::kStageSlots].ptr), piece, off,
+                 std::min(piece, n - off), host);
+  };
+  for (size_t k = 0; k < pieces; ++k) {
+    // slot k % kStageSlots is free once the host has copied out its last piece
+    while (drained + Context::kStageSlots <= k) drain(drained++);
+    const uint64_t off = k * piece, m = std::min(piece, n - off);
+    char* slot = static_cast<char*>(c.stage[k % Context::kStageSlots].ptr);
+    const void* srcs[8] = {buf.fate + off, buf.nb + off, buf.ns + off, buf.t + off,
+                           buf.x + off,    buf.y + off,  buf.z + off,  buf.path + off};
+    for (int a = 0; a < 8; ++a) {
+      HIPCHK(hipMemcpyAsync(slot, srcs[a], m * sz[a], hipMemcpyDeviceToHost, c.copy_stream));
+      slot += piece * sz[a];
+    }
+    HIPCHK(hipEventRecord(cev[k], c.copy_stream));
+  }
+  while (drained < pieces) drain(drained++);
+}
+
 RecordsRun run_records(Context& c, const optphot_sim_params& p, uint64_t n,
                        const optphot_records* host, const DevRecords* dev, RecKernel kind,
                        int block, uint32_t steps, uint64_t first_id, uint64_t chunk,
@@ -546,11 +624,7 @@ RecordsRun run_records(Context& c, const optphot_sim_params& p, uint64_t n,
   const uint64_t cap = dev ? std::min(n, limit)
                            : std::max<uint64_t>(1, std::min<uint64_t>({n, chunk, limit}));
   // copy piece; a history launch covers one piece, a wavefront launch a chunk
-  uint64_t piece = cap;
-  if (host) {
-    piece = std::max<uint64_t>(kPieceMin, (cap + 7) / 8);
-    piece = std::min<uint64_t>({(piece + 63) / 64 * 64, kStagePhotonsMax, cap});
-  }
+  const uint64_t piece = host ? stage_piece(cap) : cap;
   // Photons per launch: the wavefront kernel runs a chunk per launch; the
   // history kernel splits a large chunk into a few launches (a multiple of the
   // copy piece) so that copies start early, but each launch ends with its
@@ -615,29 +689,8 @@ RecordsRun run_records(Context& c, const optphot_sim_params& p, uint64_t n,
     const auto t0 = std::chrono::steady_clock::now();
     const Piece& pc = pieces[k];
     HIPCHK(hipEventSynchronize(cev[k]));
-    const char* src = static_cast<const char*>(c.stage[k % Context::kStageSlots].ptr);
-    char* dst[8] = {reinterpret_cast<char*>(host->fate + pc.off),
-                    reinterpret_cast<char*>(host->n_boundary + pc.off),
-                    reinterpret_cast<char*>(host->n_scatter + pc.off),
-                    reinterpret_cast<char*>(host->t + pc.off),
-                    reinterpret_cast<char*>(host->x + pc.off),
-                    reinterpret_cast<char*>(host->y + pc.off),
-                    reinterpret_cast<char*>(host->z + pc.off),
-                    reinterpret_cast<char*>(host->path + pc.off)};
-    std::vector<std::function<void()>> tasks;
-    const size_t split = pc.m >= (uint64_t{1} << 18) ? c.pool->size() + 1 : 1;
-    for (int a = 0; a < 8; ++a) {
-      const size_t bytes = pc.m * sz[a];
-      const size_t part = (bytes + split - 1) / split;
-      for (size_t b = 0; b < bytes; b += part) {
-        const size_t len = std::min(part, bytes - b);
-        char* d = dst[a] + b;
-        const char* from = src + b;
-        tasks.emplace_back([d, from, len] { std::memcpy(d, from, len); });
-      }
-      src += piece * sz[a];
-    }
-    c.pool->run(tasks);
+    scatter_slot(c, static_cast<const char*>(c.stage[k % Context::kStageSlots].ptr), piece,
+                 pc.off, pc.m, host);
     d2h_ms += ms_since(t0);
   };
 
@@ -1387,14 +1440,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
     const auto td = std::chrono::steady_clock::now();
     HIPCHK(hipMemcpyAsync(tally, c.tally.ptr, (static_cast<size_t>(T.words) + 2) * 8,
                           hipMemcpyDeviceToHost, c.stream));
-    if (lo.records) {
-      void* dst[8] = {records->fate, records->n_boundary, records->n_scatter, records->t,
-                      records->x,    records->y,          records->z,         records->path};
-      const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
-      for (int a = 0; a < 8; ++a)
-        HIPCHK(hipMemcpyAsync(dst[a], c.records[a].ptr, total * sz[a], hipMemcpyDeviceToHost,
-                              c.stream));
-    }
+    if (lo.records) stage_records_to_host(c, rec, total, records, c.ev[1], tm.alloc_ms);
     HIPCHK(hipStreamSynchronize(c.stream));
     tm.d2h_ms = ms_since(td);
     read_counters(c, counters);
