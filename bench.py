@@ -344,6 +344,12 @@ def main():
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched-events measurement (4096 events, autotuned shape, "
                          "against one history launch per event)")
+    ap.add_argument("--gensteps-only", action="store_true",
+                    help="only the generation-from-gensteps measurements (10^5 gensteps in "
+                         "the long-histories box)")
+    ap.add_argument("--calibration-only", action="store_true",
+                    help="only the calibration measurement (256 parameter sets, one graph per "
+                         "forward + loss + adjoint step)")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only run the library's VALU calibration kernel (all lanes, then odd "
                          "lanes only): a known instruction count for reading rocprofv3 VALU "
@@ -388,6 +394,12 @@ def main():
         return
     if args.batched_only:
         print(json.dumps({"batched_events": bench_batched()}))
+        return
+    if args.gensteps_only:
+        print(json.dumps({"gensteps": bench_gensteps()}))
+        return
+    if args.calibration_only:
+        print(json.dumps({"calibration": bench_calibration()}))
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
