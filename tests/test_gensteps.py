@@ -218,3 +218,27 @@ def test_gpu_gensteps_equal_oracle(backend, oracle_portable, name):
             assert t.scatter_sum == ref_t["scatter_sum"]
             if counters:
                 assert cnt.as_dict()["emitted"] == int(g["n_photons"].sum())
+
+
+@pytest.mark.gpu
+def test_records_through_several_staging_pieces(backend, oracle_portable):
+    """Records of ~7e5 photons leave the device in several staged pieces, more
+    than there are staging slots, so slots are reused; then a smaller run
+    reuses the buffers. Both equal the oracle's records byte for byte."""
+    from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = PARITY["c2_long"]
+    g = make_gensteps(700, p, photons_per_mm=200.0, seed=31, cherenkov_fraction=1.0 / 3.0)
+    n = int(g["n_photons"].sum())
+    assert n > 8 * (1 << 16)  # pieces of n / 8 photons at least: 8 pieces, 3 slots
+    print(f"[gensteps] records of {n} photons", flush=True)
+    c = TallyConfig(64, 100.0, 16)
+    _, ref = oracle_portable.gensteps(p, g, first_id=5, time_bins=64, time_max=100.0,
+                                      xy_bins=16, records=True)
+    _, r, _, _ = run_gpu_gensteps(p, c, g, first_id=5, records=True)
+    for f in FIELDS:
+        assert np.array_equal(getattr(r, f).view(np.uint8), ref[f].view(np.uint8)), f
+    m = int(g["n_photons"][:300].sum())
+    _, r2, _, _ = run_gpu_gensteps(p, c, g[:300], first_id=5, records=True)
+    for f in FIELDS:
+        assert np.array_equal(getattr(r2, f).view(np.uint8), ref[f][:m].view(np.uint8)), f
