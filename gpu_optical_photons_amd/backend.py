@@ -130,6 +130,9 @@ def _load():
                                                            ctypes.c_uint64, P(WavefrontTuning),
                                                            P(EventStats), P(StepCounters),
                                                            P(RunTiming)]),
+        "optphot_run_gpu_tally_wavefront_noalloc": (ctypes.c_int, [
+            P(SimParams), P(TallyConfig), ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64,
+            P(WavefrontTuning), P(RunTiming)]),
         "optphot_run_gpu_tally_sorted": (ctypes.c_int, [P(SimParams), P(TallyConfig),
                                                         ctypes.c_uint64, ctypes.c_void_p,
                                                         ctypes.c_uint64, P(SortedTuning),
@@ -339,6 +342,29 @@ def run_gpu_tally_wavefront(p: SimParams, c: TallyConfig, n: int, first_id: int 
         st.launches = 1
         st.kernels_ms = tm.kernel_ms
     return Tally.from_flat(flat, c), tm, st, cnt
+
+
+def run_gpu_tally_wavefront_noalloc(p: SimParams, c: TallyConfig, n: int, out, first_id: int = 0,
+                                    tuning: WavefrontTuning | dict | None = None):
+    """The wavefront tally from the entry point that allocates nothing
+    (optphot_run_gpu_tally_wavefront_noalloc; no reference counterpart): the
+    tally goes into `out`, a caller-owned device tensor of c.words 64-bit
+    words (torch.int64 or torch.uint64), and the launch works in the
+    library's static device scratch. Returns the RunTiming. Some other call of
+    the library must have set up the device first."""
+    import torch
+    if out.device.type != "cuda" or out.element_size() != 8 or out.numel() != c.words \
+            or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous device tensor of {c.words} 64-bit words")
+    if isinstance(tuning, dict):
+        tuning = WavefrontTuning(**tuning)
+    torch.cuda.synchronize(out.device)  # torch's work on `out` is done before the library's
+    tm = RunTiming()
+    rc = _lib.optphot_run_gpu_tally_wavefront_noalloc(
+        ctypes.byref(p), ctypes.byref(c), n, ctypes.c_void_p(out.data_ptr()), first_id,
+        ctypes.byref(tuning) if tuning is not None else None, ctypes.byref(tm))
+    _check(rc)
+    return tm
 
 
 def run_gpu_tally_sorted(p: SimParams, c: TallyConfig, n: int, first_id: int = 0,

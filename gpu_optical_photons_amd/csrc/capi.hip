@@ -26,6 +26,18 @@
 #include "batch_queue.hpp"
 #include "kernels.hpp"
 
+// Device scratch of optphot_run_gpu_tally_wavefront_noalloc, the entry point
+// that allocates nothing: the queue head, the block sums and the per-block
+// partial histograms of its launch. Each device holds its own copy for the
+// life of the process. Host code must never pass `scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): the
+// context looks its device address up once at set-up (hipGetSymbolAddress,
+// Context::init) and only that address is used.
+namespace noalloc_store {
+constexpr size_t kScratchBytes = size_t{16} << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+}  // namespace noalloc_store
+
 namespace {
 
 using namespace ophip;
@@ -185,6 +197,10 @@ struct Context {
   PinnedBuf sorted_status;
   hipGraphExec_t sorted_exec = nullptr;
   std::string sorted_key;
+  // Device address of noalloc_store::scratch on each device, looked up at
+  // set-up with that device current (only index `device` is used).
+  static constexpr int kMaxDevices = 64;
+  void* static_scratch[kMaxDevices] = {};
 
   // Every captured graph refers to the buffers it was built with.
   void drop_graphs() {
@@ -223,6 +239,13 @@ struct Context {
     launch_rayleigh_table(rayleigh.as<float>(), stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
+    if (device < 0 || device >= kMaxDevices)
+      throw std::runtime_error("device index " + std::to_string(device) + " above " +
+                               std::to_string(kMaxDevices - 1));
+    void* sp = nullptr;
+    HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(noalloc_store::scratch)));
+    if (!sp) throw std::runtime_error("hipGetSymbolAddress returned NULL for the static scratch");
+    static_scratch[device] = sp;
     ready = true;
   }
 
@@ -957,6 +980,81 @@ __attribute__((visibility("default"))) int optphot_run_gpu_tally_wavefront(
     optphot_run_timing tm{};
     run_tally_common(c, *p, *tc, n, tally, first_id, true, s.block, s.grid, s, tm, stats,
                      counters);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_tally_wavefront_noalloc(
+    const optphot_sim_params* p, const optphot_tally_config* tc, uint64_t n, uint64_t* dev_tally,
+    uint64_t first_id, const optphot_wavefront_tuning* tuning, optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    if (!tc || !dev_tally) throw InvalidArgument("noalloc tally: config / output is NULL");
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (!c.ready)
+      throw std::runtime_error(
+          "noalloc tally: the library is not set up on this device; make any other call first "
+          "(for example optphot_gpu_device_name), which prepares what this entry point uses");
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, dev_tally) != hipSuccess ||
+        attr.type != hipMemoryTypeDevice)
+      throw InvalidArgument("noalloc tally: the output must be device memory");
+    const DevTally T = to_dev_tally(*tc, *p);
+    const size_t lds = static_cast<size_t>(T.words) * 4;
+    check_lds(c, T);
+    WavefrontShape s = wavefront_shape(c, tuning, false, true, lds, n);
+    // Scratch layout (bytes): queue head and counters, 2 u64 sums per block,
+    // then at the next 256-byte boundary the u32 partial rows of every block.
+    constexpr size_t kCap = noalloc_store::kScratchBytes;
+    auto rows_at = [](uint64_t g) { return (kScratchSums * 8 + 16 * g + 255) & ~size_t{255}; };
+    auto bytes_for = [&](uint64_t g) { return rows_at(g) + 4ull * T.words * g; };
+    const uint64_t fit = (kCap - kScratchSums * 8 - 255) / (16 + 4ull * T.words);
+    const uint64_t want = static_cast<uint64_t>(s.grid);
+    // fewer persistent blocks than the occupancy allows only costs speed (the
+    // tally does not depend on the grid), but not fewer than one per CU
+    if (want > fit) {
+      if (fit < std::min<uint64_t>(want, static_cast<uint64_t>(c.cu_count)))
+        throw InvalidArgument(
+            "noalloc tally: a tally of " + std::to_string(T.words) + " words needs " +
+            std::to_string(bytes_for(std::min<uint64_t>(want, c.cu_count))) +
+            " bytes of scratch at one block per compute unit, above the static maximum of " +
+            std::to_string(kCap) + " bytes; use fewer xy_bins / time_bins or "
+            "optphot_run_gpu_tally_wavefront");
+      s.grid = static_cast<int>(fit);
+    }
+    if (bytes_for(static_cast<uint64_t>(s.grid)) > kCap)
+      throw std::runtime_error("noalloc tally: scratch layout exceeds its maximum");
+    if (n / static_cast<uint64_t>(s.grid) >= 0xffffffffull)
+      throw InvalidArgument("noalloc tally: too many photons for one tally launch; split the run");
+    char* base = static_cast<char*>(c.static_scratch[c.device]);
+    if (!base) throw std::runtime_error("noalloc tally: static scratch address not set up");
+    auto* words64 = reinterpret_cast<unsigned long long*>(base);
+    auto* out_tally = reinterpret_cast<unsigned long long*>(dev_tally);
+
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+    LaunchOut out{};
+    out.tally = T;
+    out.partials = reinterpret_cast<uint32_t*>(base + rows_at(static_cast<uint64_t>(s.grid)));
+    out.sums = words64 + kScratchSums;
+    out.counters = nullptr;
+    out.tally_on = true;
+    out.records = false;
+    // The array keeps the previous call's contents: clear the queue head and
+    // the block sums (each block writes its whole partial row).
+    launch_zero_u64(words64, kScratchSums + 2 * static_cast<size_t>(s.grid), c.stream);
+    launch_zero_u64(out_tally, static_cast<size_t>(T.words) + 2, c.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev[0], c.stream));
+    launch_wavefront(to_dev(*p), key_of(*p), first_id, n, s, words64 + kScratchQueue, out,
+                     c.stream);
+    HIPCHK(hipGetLastError());
+    launch_tally_reduce(T, s.grid, out.partials, out.sums, out_tally, c.stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev[1], c.stream));
+    tm.kernel_ms = c.elapsed(0, 1);
+    tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
   });
 }

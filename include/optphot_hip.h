@@ -189,6 +189,30 @@ int optphot_run_gpu_tally_wavefront(const optphot_sim_params* p, const optphot_t
                                     optphot_event_stats* stats, optphot_step_counters* counters,
                                     optphot_run_timing* timing);
 
+/* The tally of optphot_run_gpu_tally_wavefront, word for word, from a call
+ * that allocates nothing. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc). The caller passes `dev_tally`, OPTPHOT_TALLY_WORDS(*c) u64
+ * words of DEVICE memory, which receive the tally. The launch's working
+ * memory (queue head, block sums, per-block partial histograms) is one
+ * static 16 MiB device array of the library, held by each device until the
+ * process ends; its address is looked up when the library sets up a device,
+ * so some other call (for example optphot_gpu_device_name) must come first
+ * on the device, or this one fails with OPTPHOT_ERR_RUNTIME. The persistent
+ * grid is trimmed to what the array holds (results do not depend on it); a
+ * binning whose partial rows do not fit at one block per compute unit
+ * (about 16 000 tally words) is rejected with OPTPHOT_ERR_INVALID_ARGUMENT.
+ * Every call in the process on a device shares that device's array, which
+ * keeps the last call's contents (each call clears what must start at zero).
+ * Calls that overlapped on one device would race on it; the library's lock
+ * serialises calls within the process and each returns only when its
+ * kernels are done. tuning / timing may be NULL; no branch-event counters. */
+int optphot_run_gpu_tally_wavefront_noalloc(const optphot_sim_params* p,
+                                            const optphot_tally_config* c, uint64_t n,
+                                            uint64_t* dev_tally, uint64_t first_id,
+                                            const optphot_wavefront_tuning* tuning,
+                                            optphot_run_timing* timing);
+
 /* Branch-event counters of the history-based tally kernel (same work as
  * optphot_run_gpu_tally). */
 int optphot_run_gpu_tally_counters(const optphot_sim_params* p, const optphot_tally_config* c,
