@@ -2,6 +2,8 @@
 // 1.0 arrays and meta.json (file set and dtypes of src/core/io.cpp:14-74).
 // Arrays are written in the host's byte order and declared little-endian;
 // x86-64 hosts of MI355X are little-endian.
+#include <cmath>
+#include <cstdio>
 #include <filesystem>
 #include <fstream>
 #include <stdexcept>
@@ -40,9 +42,11 @@ void write_npy(const std::string& path, const void* data, const std::string& des
 }
 
 void write_text_file(const std::string& path, const std::string& text) {
-  std::ofstream f(path);
-  if (!f) throw std::runtime_error("cannot write '" + path + "'");
-  f << text << '\n';
+  std::FILE* f = std::fopen(path.c_str(), "w");
+  bool ok = f != nullptr;
+  if (ok) ok = std::fputs(text.c_str(), f) >= 0 && std::fputc('\n', f) != EOF;
+  if (f && std::fclose(f) != 0) ok = false;
+  if (!ok) throw std::runtime_error("cannot write '" + path + "'");
 }
 
 void write_records(const std::string& dir, const PhotonRecords& r, const std::string& meta) {
@@ -58,12 +62,22 @@ void write_records(const std::string& dir, const PhotonRecords& r, const std::st
 }
 
 void write_tally(const std::string& dir, const Tally& t, const std::string& meta) {
+  // the hit map is square: xy_bins x xy_bins words, row-major [iy][ix]
+  const std::size_t words = t.xy_hist.size();
+  const auto side = static_cast<std::size_t>(std::llround(std::sqrt(static_cast<double>(words))));
+  if (side * side != words)
+    throw std::runtime_error("hit map of " + std::to_string(words) + " words is not square");
+  struct Array {
+    const char* file;
+    const uint64_t* data;
+    std::vector<std::size_t> shape;
+  };
+  const Array arrays[] = {{"fate_counts", t.fate_counts.data(), {t.fate_counts.size()}},
+                          {"time_hist", t.time_hist.data(), {t.time_hist.size()}},
+                          {"xy_hist", t.xy_hist.data(), {side, side}}};
   std::filesystem::create_directories(dir);
-  write_npy(dir + "/fate_counts.npy", t.fate_counts.data(), "<u8", {t.fate_counts.size()}, 8);
-  write_npy(dir + "/time_hist.npy", t.time_hist.data(), "<u8", {t.time_hist.size()}, 8);
-  std::size_t nb = 0;
-  while (nb * nb < t.xy_hist.size()) ++nb;
-  write_npy(dir + "/xy_hist.npy", t.xy_hist.data(), "<u8", {nb, nb}, 8);
+  for (const Array& a : arrays)
+    write_npy(dir + "/" + a.file + ".npy", a.data, "<u8", a.shape, sizeof(uint64_t));
   write_text_file(dir + "/meta.json", meta);
 }
 

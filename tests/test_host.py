@@ -36,7 +36,7 @@ def test_host_unit_tests(tmp_path):
 
 def test_driver_help_and_cpu_backend_refused(tmp_path):
     r = subprocess.run([_bin("optphot"), "--help"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 0 and "size_x size_y size_z" in r.stdout
+    assert r.returncode == 0 and "size_x" in r.stdout and "scat_length" in r.stdout
     r = subprocess.run([_bin("optphot"), "--backend", "cpu"], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 2 and "GPU backend only" in r.stderr
