@@ -305,6 +305,82 @@ def bench_differentiation():
             "gradient_check_photons": nc, "gradient_vs_fd": check}
 
 
+def bench_gradient_checks(photons=4_000_000_000):
+    """The adjoint against central finite differences (common random
+    numbers) of the linearised chi-square loss, `photons` per side in 16
+    disjoint id ranges, with the error per parameter:
+      * lambertian: the calibration geometry, R = 0.97 (the reflectivity
+        enters through every wall decision);
+      * specular: a 20 x 60 x 40 mm specular box, R = 0.9;
+      * polished: the single-geometry workload with the FD step scanned over
+        relative 1e-3, 3e-4 and 1e-4 for n_det and n_scint (FD curvature
+        against estimator bias)."""
+    from gpu_optical_photons_amd import calibrate as cal
+    out = {"photons_per_side": photons}
+    for name in ("lambertian", "specular"):
+        tpl, theta, T, bins, tmax = cal.gradient_problem(name, 20_000_000)
+        out[name] = cal.gradient_check(tpl, theta, T, photons, 20_000_000, bins, tmax)
+    tpl, theta, T, bins, tmax = cal.gradient_problem("polished", 10_000_000)
+    w = cal.linearised_weights(tpl, theta, T, 10_000_000, 10_000_000, bins, tmax)
+    scan = {}
+    for step in (1e-3, 3e-4, 1e-4):
+        scan[f"{step:g}"] = cal.gradient_check(tpl, theta, T, photons, 10_000_000, bins, tmax,
+                                               rel_step=step, params=(3, 4), dl_dh=w)
+    out["polished_step_scan"] = scan
+    # The point source's emission cosines lie on a 2^-23 grid that the
+    # polished walls keep up to the readout, so the adjoint (the derivative
+    # at fixed indices) depends on where that grid falls against the critical
+    # cosine. One float32 ulp of n_det moves the critical cosine by 1.44
+    # grid cells: the mean over 16 consecutive ulps is the smooth derivative
+    # that finite differences see.
+    import numpy as np
+    nd = np.float32(theta[3])
+    per_ulp = []
+    for k in range(16):
+        th_k = theta.copy()
+        th_k[3] = float(nd)
+        g_k, _ = cal.lr_gradient(tpl, th_k, T, 1_000_000_000, 10_000_000, parts=4, dl_dh=w,
+                                 bins=bins, tmax=tmax)
+        per_ulp.append([float(g_k[3]), float(g_k[4])])
+        nd = np.nextafter(nd, np.float32(np.inf))
+    per_ulp = np.array(per_ulp)
+    fd1 = scan["0.001"]
+    out["polished_emission_grid_average"] = {
+        "n_det_ulps": 16, "photons_per_point": 1_000_000_000,
+        "adjoint_n_det": per_ulp[:, 0].tolist(), "adjoint_n_scint": per_ulp[:, 1].tolist(),
+        "mean_n_det": float(per_ulp[:, 0].mean()), "mean_n_scint": float(per_ulp[:, 1].mean()),
+        "fd_n_det": fd1["n_det"]["fd"], "fd_n_det_se": fd1["n_det"]["fd_se"],
+        "fd_n_scint": fd1["n_scint"]["fd"], "fd_n_scint_se": fd1["n_scint"]["fd_se"],
+        "mean_rel_diff_n_det": float(per_ulp[:, 0].mean() / fd1["n_det"]["fd"] - 1.0),
+        "mean_rel_diff_n_scint": float(per_ulp[:, 1].mean() / fd1["n_scint"]["fd"] - 1.0)}
+    # exact case: mirror box with a Fresnel readout (float64 quadrature;
+    # tests/golden/mirror_box_derivative.json)
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "mirror_box_derivative.json")))
+    import math
+
+    import numpy as np
+
+    from gpu_optical_photons_amd.params import SimParams
+    p = SimParams(**g["params"], abs_length=math.inf, scat_length=math.inf, surface=2,
+                  reflectivity=g["reflectivity"], tau=0.0, seed=21)
+    th = np.array([math.inf, math.inf, g["reflectivity"], 1.5, 1.63])
+    w = np.ones(9)
+    lr, lr_se = cal.lr_gradient(p, th, None, 4 * photons, 1, parts=16, dl_dh=w, bins=8, tmax=5000.0)
+    fd, fd_se = cal.fd_gradient(p, th, None, photons, 1, parts=16, dl_dh=w, bins=8, tmax=5000.0,
+                                params=(3, 4))
+    mb = {}
+    for j, key in ((3, "ndet"), (4, "nscint")):
+        smooth = g[f"dP_d{key}_smooth"]
+        quant = smooth + g[f"grid_correction_{key}"]
+        mb[key] = {"adjoint": float(lr[j]), "adjoint_se": float(lr_se[j]),
+                   "quantised_exact": quant, "adjoint_pull": float((lr[j] - quant) / lr_se[j]),
+                   "fd": float(fd[j]), "fd_se": float(fd_se[j]), "smooth_exact": smooth,
+                   "fd_pull": float((fd[j] - smooth) / fd_se[j])}
+    out["mirror_box_fresnel_exact"] = {"adjoint_photons": 4 * photons, "fd_photons_per_side": photons,
+                                       **mb}
+    return out
+
+
 def bench_calibration():
     """Batched differentiable calibration: 256 parameter sets x 1e6 photons,
     long-histories geometry, (abs_length, R, n_det) fitted by 50 Adam
@@ -347,6 +423,9 @@ def main():
     ap.add_argument("--gensteps-only", action="store_true",
                     help="only the generation-from-gensteps measurements (10^5 gensteps in "
                          "the long-histories box)")
+    ap.add_argument("--gradients-only", action="store_true",
+                    help="only the adjoint-against-finite-difference checks (4e9 photons per "
+                         "side at the Lambertian, specular and polished geometries)")
     ap.add_argument("--calibration-only", action="store_true",
                     help="only the calibration measurement (256 parameter sets, one graph per "
                          "forward + loss + adjoint step)")
@@ -397,6 +476,9 @@ def main():
         return
     if args.gensteps_only:
         print(json.dumps({"gensteps": bench_gensteps()}))
+        return
+    if args.gradients_only:
+        print(json.dumps({"gradient_checks": bench_gradient_checks()}))
         return
     if args.calibration_only:
         print(json.dumps({"calibration": bench_calibration()}))

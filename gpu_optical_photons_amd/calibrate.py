@@ -62,23 +62,26 @@ def loss_and_grad(template, theta, target, n, n_target, first_id=0):
     return loss, g[0], tf.kernel_ms, ta.kernel_ms, dl_dh
 
 
-def loss_at(template, theta, target, n, n_target, first_id=0):
+def loss_at(template, theta, target, n, n_target, first_id=0, bins=C4_BINS, tmax=C4_TMAX):
     p = adjoint.with_theta(template, theta)
-    h, _ = adjoint.histograms(EventBatch([p], np.array([n], np.uint64)), C4_BINS, C4_TMAX,
-                              first_id)
+    h, _ = adjoint.histograms(EventBatch([p], np.array([n], np.uint64)), bins, tmax, first_id)
+    if target is None:  # only the histogram is wanted
+        return float("nan"), h[0]
     return float(adjoint.chi2_loss(h[0], target, n, n_target)), h[0]
 
 
-def fd_gradient(template, theta, target, n, n_target, rel_step=1e-3, parts=4, dl_dh=None):
+def fd_gradient(template, theta, target, n, n_target, rel_step=1e-3, parts=4, dl_dh=None,
+                bins=C4_BINS, tmax=C4_TMAX, params=range(5)):
     """Central finite differences with common random numbers (same seed and
     photon ids at theta +- h), in float64 from integer tallies. Each of
     `parts` disjoint photon-id ranges gives one estimate; returns mean and
-    standard error. With dl_dh given, differentiates the linearised loss
-    sum_k dl_dh[k] h_k (the quantity the adjoint returns), which removes the
-    O(h^2) curvature of the chi-square from the comparison."""
+    standard error (parameters not in `params` stay zero). With dl_dh given,
+    differentiates the linearised loss sum_k dl_dh[k] h_k (the quantity the
+    adjoint returns), which removes the O(h^2) curvature of the chi-square
+    from the comparison."""
     m = n // parts
     est = np.zeros((parts, 5))
-    for j in range(5):
+    for j in params:
         step = rel_step * abs(theta[j])
         tp, tm_ = theta.copy(), theta.copy()
         tp[j] += step
@@ -86,8 +89,8 @@ def fd_gradient(template, theta, target, n, n_target, rel_step=1e-3, parts=4, dl
         # the step actually taken, after float32 rounding of the parameters
         dp = np.float32(tp[j]) - np.float32(tm_[j])
         for k in range(parts):
-            lp, hp = loss_at(template, tp, target, m, n_target, first_id=k * m)
-            lm, hm = loss_at(template, tm_, target, m, n_target, first_id=k * m)
+            lp, hp = loss_at(template, tp, target, m, n_target, k * m, bins, tmax)
+            lm, hm = loss_at(template, tm_, target, m, n_target, k * m, bins, tmax)
             if dl_dh is not None:
                 est[k, j] = float(dl_dh @ (hp - hm)) / float(dp)
             else:
@@ -95,7 +98,8 @@ def fd_gradient(template, theta, target, n, n_target, rel_step=1e-3, parts=4, dl
     return est.mean(axis=0), est.std(axis=0, ddof=1) / math.sqrt(parts)
 
 
-def lr_gradient(template, theta, target, n, n_target, parts=4, dl_dh=None):
+def lr_gradient(template, theta, target, n, n_target, parts=4, dl_dh=None, bins=C4_BINS,
+                tmax=C4_TMAX):
     """Adjoint gradient on `parts` disjoint id ranges: mean and standard error.
     With dl_dh given, that fixed weight vector is used for every part."""
     m = n // parts
@@ -106,10 +110,73 @@ def lr_gradient(template, theta, target, n, n_target, parts=4, dl_dh=None):
         else:
             p = adjoint.with_theta(template, theta)
             b = EventBatch([p], np.array([m], np.uint64))
-            g, _ = adjoint.run_gpu_adjoint(b, dl_dh[None, :], C4_BINS, C4_TMAX, k * m)
+            g, _ = adjoint.run_gpu_adjoint(b, dl_dh[None, :], bins, tmax, k * m)
             g = g[0]
         est[k] = g
     return est.mean(axis=0), est.std(axis=0, ddof=1) / math.sqrt(parts)
+
+
+def linearised_weights(template, theta, target, n, n_target, bins, tmax):
+    """dloss/dh of the chi-square at theta (the adjoint's weight vector)."""
+    _, h = loss_at(template, theta, target, n, n_target, 0, bins, tmax)
+    f, F = h.sum(), target.sum()
+    wk = 1.0 / (target + 1.0 / n_target)
+    wf = 1.0 / (F * (1.0 - F) + 1.0 / n_target)
+    return 2.0 * (h - target) * wk + 2.0 * (f - F) * wf
+
+
+def gradient_check(template, theta, target, n, n_target, bins, tmax, rel_step=1e-3, parts=16,
+                   params=range(5), dl_dh=None):
+    """Adjoint against central FD (common random numbers) of the linearised
+    loss, per parameter: values, standard errors, relative difference and
+    pull (difference over the combined standard error)."""
+    if dl_dh is None:
+        dl_dh = linearised_weights(template, theta, target, min(n, 10_000_000), n_target, bins,
+                                   tmax)
+    lr, lr_se = lr_gradient(template, theta, target, n, n_target, parts, dl_dh, bins, tmax)
+    fd, fd_se = fd_gradient(template, theta, target, n, n_target, rel_step, parts, dl_dh, bins,
+                            tmax, params)
+    out = {}
+    for j in params:
+        se = math.hypot(lr_se[j], fd_se[j])
+        out[adjoint.THETA[j]] = {
+            "adjoint": float(lr[j]), "adjoint_se": float(lr_se[j]), "fd": float(fd[j]),
+            "fd_se": float(fd_se[j]),
+            "rel_diff": float((lr[j] - fd[j]) / abs(fd[j])) if fd[j] else 0.0,
+            "pull": float((lr[j] - fd[j]) / se) if se else 0.0}
+    return out
+
+
+def gradient_problem(name: str, n_target: int, seed: int = 1, target_seed: int = 777):
+    """(template, nominal theta, target histogram, bins, tmax) of a gradient
+    check. "polished": the single-geometry workload above (R does not enter).
+    "lambertian": the calibration geometry (100 x 100 x 50 mm, Lambertian
+    walls R = 0.97, Rayleigh 300 mm, volume source), 128 bins on [0, 100) ns.
+    "specular": a 20 x 60 x 40 mm box with specular walls R = 0.9, Rayleigh
+    150 mm, n_det 1.45, volume source, 128 bins on [0, 60) ns. Targets come
+    from other parameters under another seed."""
+    if name == "polished":
+        tpl, T = c4_problem(n_target, n_target, seed, target_seed)
+        return tpl, C4_NOMINAL.copy(), T, C4_BINS, C4_TMAX
+    if name == "lambertian":
+        tpl = c5_template(seed)
+        theta = np.array([2000.0, 300.0, 0.97, 1.5, 1.63])
+        target = np.array([1500.0, 400.0, 0.95, 1.45, 1.60])
+        bins, tmax = 128, 100.0
+        tgt_tpl = c5_template(target_seed)
+    elif name == "specular":
+        tpl = SimParams(hx=10.0, hy=30.0, hz=20.0, n_scint=1.63, n_out=1.0,
+                        surface=Surface.SPECULAR, source=Source.VOLUME, tau=4.0,
+                        max_steps=10000, seed=seed)
+        theta = np.array([1000.0, 150.0, 0.9, 1.45, 1.63])
+        target = np.array([1300.0, 200.0, 0.85, 1.5, 1.60])
+        bins, tmax = 128, 60.0
+        tgt_tpl = tpl.copy(seed=target_seed)
+    else:
+        raise ValueError(f"unknown gradient problem {name!r}")
+    T, _ = adjoint.histograms(EventBatch([adjoint.with_theta(tgt_tpl, target)],
+                                         np.array([n_target], np.uint64)), bins, tmax)
+    return tpl, theta, T[0], bins, tmax
 
 
 # ---------------------------------------------------------------------------

@@ -775,6 +775,20 @@ void validate_batch(const optphot_sim_params* params, const uint64_t* photons, u
   }
 }
 
+// The adjoint's reflector score is hits / R per detected photon, so every
+// set with reflecting faces needs R in (0, 1] (R = 0 would make the score of
+// a photon that met no reflector 0 / 0).
+void validate_adjoint_sets(const optphot_sim_params* params, uint32_t n_sets) {
+  for (uint32_t e = 0; e < n_sets; ++e) {
+    const optphot_sim_params& q = params[e];
+    const bool reflector = q.surface == OPTPHOT_SURFACE_SPECULAR ||
+                           q.surface == OPTPHOT_SURFACE_LAMBERTIAN;
+    if (reflector && !(q.reflectivity > 0.0f && q.reflectivity <= 1.0f))
+      throw InvalidArgument("adjoint: set " + std::to_string(e) +
+                            " has reflecting faces and reflectivity outside (0, 1]");
+  }
+}
+
 template <class F>
 int guarded(F&& f) {
   try {
@@ -1558,6 +1572,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
       throw InvalidArgument("chi2 step: NULL argument");
     validate_batch(params, photons, n_sets, time_bins, time_max,
                    reinterpret_cast<uint32_t*>(hist), OPTPHOT_BATCH_WAVEFRONT);
+    validate_adjoint_sets(params, n_sets);
     for (uint32_t e = 0; e < n_sets; ++e)
       if (!(target_photons[e] > 0.0))
         throw InvalidArgument("chi2 step: target_photons must be > 0");
@@ -1759,6 +1774,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_adjoint(
     if (!weights || !grad) throw InvalidArgument("adjoint: NULL weights / grad");
     validate_batch(params, photons, n_events, time_bins, time_max,
                    reinterpret_cast<uint32_t*>(grad), OPTPHOT_BATCH_WAVEFRONT);
+    validate_adjoint_sets(params, n_events);
     optphot_wavefront_tuning tu{256, 2, 256, 0, 0, 0};
     if (tuning) {
       if (tuning->block_size) tu.block_size = tuning->block_size;

@@ -717,7 +717,15 @@ __device__ __forceinline__ float fresnel_prob(const DevParams& P, bool readout, 
   const float rp = fast ? div_n(a_p, b_p) : a_p / b_p;
   const float R = 0.5f * (rs * rs + rp * rp);
   const bool tir = sin2_t >= 1.0f;
-  scored = !matched && !tir;
+  // The score follows the exact geometry: a decision that float32 rounds into
+  // total internal reflection (sin2_t within its rounding error of 1, so a
+  // certain reflection here) is still scored when cos_t^2 > 0 in float64
+  // (fresnel_score returns zero otherwise). Such grazing-critical hits carry
+  // the 1/cos_t peak of dR/dn; leaving them out biases the gradient low
+  // wherever photons revisit the readout. float32 sin2_t is within about six
+  // rounding errors (< 2^-21) of its exact value, so a margin of 2^-18
+  // keeps every such hit.
+  scored = !matched && sin2_t < 1.0f + 0x1p-18f;
   return matched ? 0.0f : (tir ? 1.0f : R);
 }
 

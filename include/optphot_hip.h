@@ -292,7 +292,9 @@ int optphot_run_gpu_batch(const optphot_sim_params* params, const uint64_t* phot
  * ids) by likelihood-ratio scores of every absorb / scatter / Fresnel /
  * reflector decision and, for n_scint, the pathwise arrival-time term
  * through the exponential emission time (requires tau > 0 for that term).
- * Histories are recomputed from their Philox counters (no stored tape). */
+ * Histories are recomputed from their Philox counters (no stored tape).
+ * Sets with specular or Lambertian faces need reflectivity in (0, 1] (the
+ * reflector score is hits / R); others are rejected as invalid arguments. */
 int optphot_run_gpu_adjoint(const optphot_sim_params* params, const uint64_t* photons,
                             uint32_t n_events, uint64_t first_id, uint32_t time_bins,
                             float time_max, const double* weights, double* grad,
@@ -347,7 +349,8 @@ int optphot_run_gpu_gensteps(const optphot_sim_params* p, const optphot_genstep*
  * (f64 [n_sets]) and grad = dloss_e/dtheta_j (f64 [n_sets][5], theta as for
  * optphot_run_gpu_adjoint). Parameters, offsets and targets are uploaded
  * before each replay, so a replay with new parameters is the same
- * computation as a fresh capture. Tunings may be NULL. */
+ * computation as a fresh capture. Reflectivity as for optphot_run_gpu_adjoint.
+ * Tunings may be NULL. */
 int optphot_run_gpu_chi2_step(const optphot_sim_params* params, const uint64_t* photons,
                               uint32_t n_sets, uint64_t first_id, uint32_t time_bins,
                               float time_max, const double* target, const double* target_photons,

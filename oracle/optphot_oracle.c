@@ -387,13 +387,14 @@ typedef struct {
 /* d log P(decision) / d n1, d n2 for a Fresnel decision at cos_i (float
    inputs of the FP32 history), evaluated in double from
    R = (rs^2 + rp^2)/2, cos_t = sqrt(1 - (n1/n2)^2 (1 - cos_i^2)).
-   No score for matched interfaces or total internal reflection (as decided
-   by the FP32 history). */
+   No score for matched interfaces or total internal reflection, as decided
+   in double: a hit that float32 rounds into total internal reflection (a
+   certain reflection in the history) but that has cos_t^2 > 0 in double is
+   scored as reflected. Near the critical angle dR/dn grows like 1/cos_t, so
+   those few hits carry a finite share of the derivative. */
 static void fresnel_score(float n1f, float n2f, float cif, int reflected, double* d1, double* d2) {
   *d1 = *d2 = 0.0;
   if (n1f == n2f) return;
-  const float eta = n1f / n2f;
-  if (eta * eta * fmaxf(0.0f, 1.0f - cif * cif) >= 1.0f) return; /* FP32 says TIR */
   const double n1 = n1f, n2 = n2f, ci = cif;
   const double s2i = fmax(0.0, 1.0 - ci * ci);
   const double s2t = (n1 / n2) * (n1 / n2) * s2i;

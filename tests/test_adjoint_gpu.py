@@ -118,11 +118,16 @@ def test_chi2_step_graph_equals_separate_calls(backend, n_sets, ftu):
     tpl = c5_template(3)
     rng = np.random.default_rng(n_sets)
     base = np.array([2000.0, 300.0, 0.97, 1.5, 1.63])
-    targets = EventBatch([adjoint.with_theta(c5_template(99), base * (1 + 0.05 * rng.normal(size=5)))
-                          for _ in range(n_sets)], np.full(n_sets, int(nt), np.uint64))
+    def draw(scale):
+        th = base * (1 + scale * rng.normal(size=5))
+        th[2] = min(th[2], 0.995)  # a reflectivity, at most 1
+        return th
+    targets = EventBatch([adjoint.with_theta(c5_template(99), draw(0.05)) for _ in range(n_sets)],
+                         np.full(n_sets, int(nt), np.uint64))
     T, _ = adjoint.histograms(targets, bins, tmax)
     for rep in range(3):
         th = base * (1 + 0.04 * rng.normal(size=(n_sets, 5)))
+        th[:, 2] = np.minimum(th[:, 2], 0.995)  # a reflectivity, at most 1
         batch = EventBatch([adjoint.with_theta(tpl, r) for r in th],
                            np.full(n_sets, n, np.uint64))
         print(f"[chi2 step] sets {n_sets} replay {rep}", flush=True)
@@ -137,3 +142,72 @@ def test_chi2_step_graph_equals_separate_calls(backend, n_sets, ftu):
         dl_dh = 2.0 * (h - T) * wk + 2.0 * (f - F) * wf
         g, _ = adjoint.run_gpu_adjoint(batch, dl_dh, bins, tmax)
         np.testing.assert_allclose(grad, g, rtol=1e-9, atol=1e-12 * np.abs(g).max())
+
+
+def test_reflector_gradient_against_mirror_box_closed_form(backend):
+    """dP_det/dR on the GPU against <k R^(k-1)> of the unfolded mirror box
+    (specular walls, matched readout, no bulk processes, point source): the
+    closed form of tests/analytic.py, no estimator involved."""
+    from gpu_optical_photons_amd.adjoint import run_gpu_adjoint
+    from gpu_optical_photons_amd.batch import EventBatch
+    from tests import analytic
+    from tests.test_adjoint_reflector import mirror_box
+    for R in (0.97, 0.75):
+        p = mirror_box(R)
+        n, parts, tb, tmax = 100_000_000, 4, 8, 2000.0
+        m = n // parts
+        gs = np.array([run_gpu_adjoint(EventBatch([p], np.array([m], np.uint64)),
+                                       np.ones((1, tb + 1)), tb, tmax, k * m)[0][0]
+                       for k in range(parts)])
+        g, se = gs.mean(axis=0), gs.std(axis=0, ddof=1) / math.sqrt(parts)
+        _, dP = analytic.mirror_box_detection(p, R)
+        print(f"[mirror box R={R}] GPU {g[2]:.6f} +- {se[2]:.6f}, closed form {dP:.6f}",
+              flush=True)
+        assert abs(g[2] - dP) < 4.0 * se[2] + 1e-4 * dP
+        assert se[2] < 0.002 * dP
+
+
+@pytest.mark.parametrize("name", ["lambertian", "specular"])
+def test_gradient_fd_at_reflecting_geometries(backend, name):
+    """Adjoint against central FD (relative step 1e-3, common random numbers)
+    of the linearised chi-square loss for all five parameters where the
+    reflectivity enters: the calibration geometry (Lambertian R = 0.97) and a
+    specular box (R = 0.9). 8e8 photons per side in 8 disjoint id ranges;
+    every parameter within 4 combined standard errors."""
+    from gpu_optical_photons_amd import calibrate as cal
+    n = 800_000_000
+    tpl, theta, T, bins, tmax = cal.gradient_problem(name, 20_000_000)
+    chk = cal.gradient_check(tpl, theta, T, n, 20_000_000, bins, tmax, parts=8)
+    for k, v in chk.items():
+        print(f"[gradcheck {name}] {k:12s} adjoint {v['adjoint']: .6e} +- {v['adjoint_se']:.1e} "
+              f"FD {v['fd']: .6e} +- {v['fd_se']:.1e} pull {v['pull']: .2f}", flush=True)
+        assert abs(v["pull"]) < 4.0, k
+    assert chk["reflectivity"]["adjoint"] != 0.0
+    assert abs(chk["reflectivity"]["rel_diff"]) < 0.02
+
+
+def test_index_gradients_near_the_critical_angle(backend):
+    """Mirror box with a Fresnel readout (tests/test_adjoint_critical.py):
+    the adjoint at 1.6e10 photons against the derivative of the simulated
+    model (point-source emission cosines on their 2^-23 grid), and central
+    finite differences (step 1e-3, common random numbers, 4e9 photons per
+    side) against the smooth derivative, both by float64 quadrature."""
+    from gpu_optical_photons_amd import calibrate as cal
+    from tests.test_adjoint_critical import golden, mirror_box_fresnel
+    g = golden()
+    p = mirror_box_fresnel()
+    theta = np.array([math.inf, math.inf, g["reflectivity"], 1.5, 1.63])
+    bins, tmax = 8, 5000.0
+    w = np.ones(bins + 1)
+    lr, lr_se = cal.lr_gradient(p, theta, None, 16_000_000_000, 1, parts=16, dl_dh=w, bins=bins,
+                                tmax=tmax)
+    fd, fd_se = cal.fd_gradient(p, theta, None, 4_000_000_000, 1, rel_step=1e-3, parts=16,
+                                dl_dh=w, bins=bins, tmax=tmax, params=(3, 4))
+    for j, key in ((3, "ndet"), (4, "nscint")):
+        smooth = g[f"dP_d{key}_smooth"]
+        quantised = smooth + g[f"grid_correction_{key}"]
+        print(f"[critical angle] {key}: adjoint {lr[j]:.5f} +- {lr_se[j]:.5f} (quantised "
+              f"{quantised:.5f}); FD {fd[j]:.5f} +- {fd_se[j]:.5f} (smooth {smooth:.5f})",
+              flush=True)
+        assert abs(lr[j] - quantised) < 4.0 * lr_se[j] + 1e-4 * abs(quantised)
+        assert abs(fd[j] - smooth) < 4.0 * fd_se[j] + 1e-4 * abs(smooth)

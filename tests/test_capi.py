@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from gpu_optical_photons_amd import backend
@@ -63,3 +64,22 @@ def test_no_device_reports_cleanly():
     assert backend.gpu_device_name() == "none"
     with pytest.raises(RuntimeError, match="no HIP device"):
         backend.run_gpu_records(SimParams(), 10)
+
+
+def test_adjoint_rejects_reflectivity_outside_unit_interval():
+    """The reflector score is hits / R: sets with reflecting faces need R in
+    (0, 1], checked before the device is touched."""
+    from gpu_optical_photons_amd import adjoint
+    from gpu_optical_photons_amd.batch import EventBatch
+    for surface in (2, 3):
+        for R in (0.0, -0.1, 1.5, float("nan")):
+            b = EventBatch([SimParams(surface=surface, reflectivity=R)], np.array([10], np.uint64))
+            with pytest.raises(ValueError, match="reflectivity"):
+                adjoint.run_gpu_adjoint(b, np.ones((1, 9)), 8, 50.0)
+            with pytest.raises(ValueError, match="reflectivity"):
+                adjoint.run_gpu_chi2_step(b, np.full((1, 9), 0.01), 100.0, 8, 50.0)
+    # polished and black faces never use R
+    b = EventBatch([SimParams(surface=0, reflectivity=0.0)], np.array([10], np.uint64))
+    if not backend.gpu_available():
+        with pytest.raises(RuntimeError, match="no HIP device"):
+            adjoint.run_gpu_adjoint(b, np.ones((1, 9)), 8, 50.0)
