@@ -392,7 +392,8 @@ def bench_calibration():
                               "p90_rel_error", "graph_ms", "forward_only_ms", "wall_s",
                               "forward_photons_per_s", "fwd_adj_photons_per_s",
                               "fwd_adj_over_fwd", "forward_tuning")} | {
-        "final_median_loss": r["losses"][-1], "first_median_loss": r["losses"][0]}
+        "final_median_loss": r["losses"][-1], "first_median_loss": r["losses"][0],
+        "diagnostics": cal.calibration_diagnostics(r)}
 
 
 def load_traffic():

@@ -207,28 +207,32 @@ def c5_batch(n_sets=256, photons=1_000_000, draw_seed=5, seed=1) -> EventBatch:
 def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=0.08,
               draw_seed=5, seed=1, target_seed=99, target_photons=None, log=None,
               time_forward=True):
-    """Fits (abs_length, R, n_det) of every set to its own target histogram.
-    Returns a dict with hidden / fitted parameters, errors and timings; with
-    time_forward, each iteration also times a forward-only batch at the same
-    parameters (outside the fit) for the forward+adjoint / forward ratio. The
-    forward launches take the autotuned batch shape for this batch class, if
-    any."""
+    """Fits (abs_length, R, n_det) of every set to its own target histogram
+    by Adam on normalised coordinates (each parameter linear over its range),
+    starting at the middle of each range. Returns a dict with hidden / fitted parameters,
+    errors and timings; with time_forward, each iteration also times a
+    forward-only batch at the same parameters (outside the fit) for the
+    forward+adjoint / forward ratio. The forward launches take the autotuned
+    batch shape for this batch class, if any."""
     from . import autotune
     rng = np.random.default_rng(draw_seed)
     hidden = LO + (HI - LO) * rng.random((n_sets, 3))
     target_photons = target_photons or photons
     base = np.array([2000.0, 300.0, 0.97, 1.5, 1.63])
 
-    def theta_of(x):  # x: normalised [n_sets, 3] in [0, 1]
+    def theta_rows(p):  # p: [n_sets, 3] (abs_length, R, n_det)
         th = np.tile(base, (n_sets, 1))
-        th[:, list(FIT)] = LO + (HI - LO) * x
+        th[:, list(FIT)] = p
         return th
 
     # targets at the hidden parameters, held-out seed
     tpl_t = c5_template(target_seed)
-    tb = EventBatch([adjoint.with_theta(tpl_t, th) for th in theta_of((hidden - LO) / (HI - LO))],
+    tb = EventBatch([adjoint.with_theta(tpl_t, th) for th in theta_rows(hidden)],
                     np.full(n_sets, target_photons, np.uint64))
     T, _ = adjoint.histograms(tb, bins, tmax)
+
+    def theta_of(x):
+        return theta_rows(LO + (HI - LO) * x)
 
     x = np.full((n_sets, 3), 0.5)
     m = np.zeros_like(x)
@@ -275,4 +279,84 @@ def calibrate(n_sets=256, photons=1_000_000, iters=50, bins=128, tmax=100.0, lr=
         "fwd_adj_photons_per_s": total / (step_ms * 1e-3),
         "fwd_adj_over_fwd": step_ms / fwd_ms if fwd_ms else None,
         "forward_tuning": tuning,
+        "targets": T, "target_photons": target_photons, "bins": bins, "tmax": tmax,
+        "seed": seed, "draw_seed": draw_seed, "lr": lr,
+    }
+
+
+def _theta_rows(p, n_sets):
+    """Full theta rows with (abs_length, R, n_det) = p [E, 3]."""
+    th = np.tile(np.array([2000.0, 300.0, 0.97, 1.5, 1.63]), (n_sets, 1))
+    th[:, list(FIT)] = p
+    return th
+
+
+def step_at(x, T, target_photons, photons, bins, tmax, seed=1, tuning=None):
+    """(loss [E], gradient in normalised coordinates [E, 3]) of every set at
+    normalised parameters x [E, 3], from one calibration-step graph."""
+    n_sets = len(x)
+    tpl = c5_template(seed)
+    batch = EventBatch([adjoint.with_theta(tpl, r)
+                        for r in _theta_rows(LO + (HI - LO) * x, n_sets)],
+                       np.full(n_sets, photons, np.uint64))
+    _, loss, g, _ = adjoint.run_gpu_chi2_step(batch, T, target_photons, bins, tmax,
+                                              forward_tuning=tuning)
+    return loss, g[:, list(FIT)] * (HI - LO)
+
+
+def hessian_at(x, T, target_photons, photons, bins, tmax, seed=1, tuning=None, h=0.02):
+    """Loss Hessian [E, 3, 3] in normalised coordinates at x, by central
+    differences of the adjoint gradient with common random numbers."""
+    H = np.zeros((len(x), 3, 3))
+    for j in range(3):
+        xp, xm = x.copy(), x.copy()
+        xp[:, j] += h
+        xm[:, j] -= h
+        _, gp = step_at(xp, T, target_photons, photons, bins, tmax, seed, tuning)
+        _, gm = step_at(xm, T, target_photons, photons, bins, tmax, seed, tuning)
+        H[:, :, j] = (gp - gm) / (2.0 * h)
+    return 0.5 * (H + np.transpose(H, (0, 2, 1)))
+
+
+def calibration_diagnostics(r, h=0.02, long_iters=300, log=None):
+    """Why each fitted parameter ends where it does, from a calibrate()
+    result r (medians over the sets):
+      * loss at the fitted and at the hidden parameters (same simulation
+        seed);
+      * the same fit run for long_iters iterations: once its loss is at the
+        hidden parameters' level (the noise floor of the data), its remaining
+        error is what the data cannot resolve;
+      * the Hessian of the loss at the hidden parameters in normalised
+        coordinates, by central differences of the adjoint gradient with
+        common random numbers (step h): eigenvalues and the abs_length share
+        of the softest direction."""
+    n_sets, photons = r["n_sets"], r["photons_per_set"]
+    T, nt, bins, tmax = r["targets"], r["target_photons"], r["bins"], r["tmax"]
+    tu, seed = r["forward_tuning"], r["seed"]
+
+    def x_of(p):
+        return (p - LO) / (HI - LO)
+
+    loss_h, _ = step_at(x_of(r["hidden"]), T, nt, photons, bins, tmax, seed, tu)
+    loss_f, _ = step_at(x_of(r["fitted"]), T, nt, photons, bins, tmax, seed, tu)
+    H = hessian_at(x_of(r["hidden"]), T, nt, photons, bins, tmax, seed, tu, h)
+    ev, evec = np.linalg.eigh(H)
+    r2 = calibrate(n_sets=n_sets, photons=photons, iters=long_iters, bins=bins, tmax=tmax,
+                   lr=r["lr"], draw_seed=r["draw_seed"], seed=seed, target_photons=nt,
+                   time_forward=False, log=log)
+    loss_l, _ = step_at(x_of(r2["fitted"]), T, nt, photons, bins, tmax, seed, tu)
+    names = ("abs_length", "reflectivity", "n_det")
+    return {
+        "loss_fitted_over_hidden_median": float(np.median(loss_f / loss_h)),
+        "long_fit_iterations": long_iters,
+        "long_fit_loss_over_hidden_median": float(np.median(loss_l / loss_h)),
+        "long_fit_fraction_loss_le_hidden": float(np.mean(loss_l <= loss_h)),
+        "long_fit_median_rel_error": r2["median_rel_error"],
+        "long_fit_p90_rel_error": r2["p90_rel_error"],
+        "hessian_diag_median": dict(zip(names, np.median(np.diagonal(H, axis1=1, axis2=2),
+                                                          axis=0).tolist())),
+        "hessian_eigenvalues_median": np.median(ev, axis=0).tolist(),
+        "hessian_condition_median": float(np.median(ev[:, 2] / np.maximum(ev[:, 0], 1e-30))),
+        "softest_direction_abs_component_median": float(np.median(np.abs(evec[:, 0, 0]))),
+        "hessian_step": h,
     }
