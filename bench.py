@@ -77,6 +77,23 @@ def cpu_baseline(p, c):
                            "sample": f"photon ids 0..{n1 - 1}, {d1:.1f} s wall"}}
 
 
+def _cpu_lib():
+    """The reference's CPU path (oracle/_ref, its default libm build), or the
+    oracle's port where that is not built. Returns (library, kind)."""
+    from oracle.oracle import Oracle, Reference, reference_available
+    if reference_available("libm"):
+        return Reference("libm"), "reference"
+    return Oracle("libm"), "port"
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        return ""
+
+
 def bench_plain_small():
     """Plain method, small: 50 mm cube, polished faces to n_out 1.0, readout
     n_det 1.5, abs 1000 mm, no Rayleigh, point source at the centre, tau 4 ns,
@@ -84,11 +101,13 @@ def bench_plain_small():
     records (history kernel, one lane per photon; wavefront kernel); the
     end-to-end rate of records delivered into host memory (kernel pieces,
     pinned staging, D2H and host copies overlapped); all records identical
-    bit for bit."""
+    bit for bit. Roofline from the branch counters of the same photons; the
+    reference's CPU path on the same photons (records and tally, all cores
+    and one thread)."""
     import numpy as np
 
-    from gpu_optical_photons_amd import backend
-    from gpu_optical_photons_amd.params import SimParams
+    from gpu_optical_photons_amd import backend, roofline
+    from gpu_optical_photons_amd.params import SimParams, TallyConfig
     p = SimParams(n_det=1.5, abs_length=1000.0, seed=1)
     n = 1_000_000
     backend.run_gpu_records_device(p, n, 0)
@@ -105,6 +124,10 @@ def bench_plain_small():
                                   dev_w[f].cpu().numpy().view(np.uint8))
                for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"))
     steps = int(rec.steps().sum())
+    # branch counts of the same photons (history tally, counting variant)
+    c = TallyConfig()
+    tally, _, cnt = backend.run_gpu_tally(p, c, n, counters=True)
+    ops = roofline.algorithmic_ops(p, cnt.as_dict(), tally.fate_counts)
     # the same pipeline at 10x the photons, into fresh arrays (pages first
     # touched by the copies) and into arrays a caller reuses
     n10 = 10 * n
@@ -114,16 +137,40 @@ def bench_plain_small():
     for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"):
         getattr(out10, f).fill(0)
     reused10 = min(backend.run_gpu_records(p, n10, out=out10)[1].total_ms for _ in range(2))
+    # CPU: the reference's records and tally runs on the same photons
+    lib, kind = _cpu_lib()
+    cores = int(lib.max_threads())
+    cpu = {}
+    for name, fn in (("records", lambda th: lib.records(p, n, threads=th)),
+                     ("tally", lambda th: lib.tally(p, n, c.time_bins, c.time_max, c.xy_bins,
+                                                    threads=th))):
+        fn(0)
+        t0 = time.perf_counter()
+        fn(0)
+        dt = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        fn(1)
+        d1 = time.perf_counter() - t1
+        cpu[name] = {"value": n / dt, "unit": "photons/s", "cores": cores,
+                     "one_thread_photons_per_s": n / d1}
     return {"photons": n, "history_kernel_ms": tm.kernel_ms,
             "history_photons_per_s": n / (tm.kernel_ms * 1e-3),
             "history_photon_steps_per_s": steps / (tm.kernel_ms * 1e-3),
             "wavefront_kernel_ms": tme.kernel_ms,
             "wavefront_photons_per_s": n / (tme.kernel_ms * 1e-3),
+            "wavefront_photon_steps_per_s": steps / (tme.kernel_ms * 1e-3),
             "wavefront_simt_efficiency": st.simt_efficiency(),
+            "roofline": roofline.roofline(ops, tm.kernel_ms, "history_kernel (records)"),
+            "roofline_wavefront": roofline.roofline(ops, tme.kernel_ms,
+                                                    "wavefront_kernel (records)"),
+            "records_write_GBps": n * 29 / (tm.kernel_ms * 1e-3) / 1e9,
             "records_to_host_total_ms": total_ms, "records_to_host_d2h_ms": d2h_ms,
             "records_to_host_photons_per_s": n / (total_ms * 1e-3),
             "records_to_host_GBps": n * 29 / (total_ms * 1e-3) / 1e9,
             "records_identical": bool(same), "photon_steps": steps,
+            "cpu_baseline": {"kind": kind, "cores": cores, "nproc": os.cpu_count(),
+                             "cpu_model": _cpu_model(), "sample": f"photon ids 0..{n - 1}",
+                             **cpu},
             "records_to_host_10x": {
                 "photons": n10, "fresh_arrays_ms": fresh10, "reused_arrays_ms": reused10,
                 "fresh_GBps": n10 * 29 / (fresh10 * 1e-3) / 1e9,
@@ -255,20 +302,50 @@ def bench_batched():
     N ~ Poisson(1e4 E), E ~U[0.1,3] MeV; generator seed 2024, run seed 1),
     per-event detected count + 128-bin time histogram 0-50 ns. One persistent
     graph-captured launch (the autotuned batch shape of this batch class, if
-    tuning.json holds one) vs one history-kernel launch per event."""
+    tuning.json holds one) vs one history-kernel launch per event; roofline
+    from the branch counters of the same photons; the CPU baseline is the
+    reference's CPU tally run per event with the same photon-id offsets."""
     import numpy as np
 
-    from gpu_optical_photons_amd import autotune
+    from gpu_optical_photons_amd import autotune, roofline
     from gpu_optical_photons_amd.batch import PLAIN, WAVEFRONT, make_events, run_gpu_batch
     b = make_events(4096, seed=2024)
     tu = autotune.batch_choice_for(b, 128)
     run_gpu_batch(b, 128, 50.0, method=WAVEFRONT, tuning=tu)  # capture
-    w, tw, _ = run_gpu_batch(b, 128, 50.0, method=WAVEFRONT, tuning=tu)
+    ms = []
+    for _ in range(5):
+        w, tw, _ = run_gpu_batch(b, 128, 50.0, method=WAVEFRONT, tuning=tu)
+        ms.append(tw.kernel_ms)
+    w_ms = float(np.median(ms))
     pl, tp, _ = run_gpu_batch(b, 128, 50.0, method=PLAIN)
-    return {"events": b.n_events, "photons": b.total, "tuning": tu, "wavefront_ms": tw.kernel_ms,
-            "plain_ms": tp.kernel_ms, "wavefront_photons_per_s": b.total / (tw.kernel_ms * 1e-3),
+    wc, _, cnt = run_gpu_batch(b, 128, 50.0, method=WAVEFRONT, tuning=tu, counters=True)
+    counters = cnt.as_dict()
+    fates = w[:, :5].sum(axis=0, dtype=np.uint64)
+    volume = int(sum(int(n) for q, n in zip(b.params, b.photons) if q.source == 1))
+    ops = roofline.algorithmic_ops(None, counters, fates, emitted_volume=volume)
+    steps = roofline.photon_steps(counters)
+    kernel = "batch_ring_kernel" if (tu or {}).get("stream_in_lds") else "batch_kernel"
+    # CPU: the reference's tally of every event, photon ids at the event's offset
+    lib, kind = _cpu_lib()
+    cores = int(lib.max_threads())
+    offs = b.offsets
+    t0 = time.perf_counter()
+    for e in range(b.n_events):
+        lib.tally(b.params[e], int(b.photons[e]), 128, 50.0, 1, first_id=int(offs[e]), threads=0)
+    dt = time.perf_counter() - t0
+    return {"events": b.n_events, "photons": b.total, "tuning": tu, "wavefront_ms": w_ms,
+            "wavefront_ms_runs": ms, "plain_ms": tp.kernel_ms,
+            "wavefront_photons_per_s": b.total / (w_ms * 1e-3),
+            "wavefront_photon_steps_per_s": steps / (w_ms * 1e-3),
             "plain_photons_per_s": b.total / (tp.kernel_ms * 1e-3),
-            "identical": bool(np.array_equal(w, pl))}
+            "plain_photon_steps_per_s": steps / (tp.kernel_ms * 1e-3),
+            "simt_efficiency": counters["useful_lane_steps"] / counters["issued_lane_steps"],
+            "roofline": roofline.roofline(ops, w_ms, kernel),
+            "identical": bool(np.array_equal(w, pl) and np.array_equal(w, wc)),
+            "cpu_baseline": {"value": b.total / dt, "unit": "photons/s", "cores": cores,
+                             "kind": kind, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                             "sample": f"all {b.n_events} events, one tally call per event "
+                                       f"({dt:.1f} s wall)"}}
 
 
 def bench_differentiation():
@@ -277,17 +354,31 @@ def bench_differentiation():
     R 0.95, n_det 1.45, n_scint 1.60), seed 777. Forward (tally) + adjoint
     (likelihood-ratio + pathwise) timings, and the gradient against central
     finite differences (relative step 1e-3, common random numbers) of the
-    linearised loss on 4e7 photons in four disjoint id ranges."""
+    linearised loss on 4e9 photons in 16 disjoint id ranges. Roofline of the
+    forward launch from its branch counters (the adjoint recomputes the same
+    transport; its scores are not counted, so its figure is a lower bound).
+    CPU: the reference's forward tally and the oracle's FP64 gradient."""
     import math
 
     import numpy as np
 
+    from gpu_optical_photons_amd import adjoint, roofline
     from gpu_optical_photons_amd import calibrate as cal
+    from gpu_optical_photons_amd.batch import EventBatch, run_gpu_batch
     n = 10_000_000
     tpl, T = cal.c4_problem(n)
     theta = cal.C4_NOMINAL.copy()
     cal.loss_and_grad(tpl, theta, T, n, n)  # warm
-    loss, g, f_ms, a_ms, dl_dh = cal.loss_and_grad(tpl, theta, T, n, n)
+    runs = [cal.loss_and_grad(tpl, theta, T, n, n) for _ in range(3)]
+    f_ms = float(np.median([r[2] for r in runs]))
+    a_ms = float(np.median([r[3] for r in runs]))
+    loss, g, _, _, dl_dh = runs[0]
+    p = adjoint.with_theta(tpl, theta)
+    counts, _, cnt = run_gpu_batch(EventBatch([p], np.array([n], np.uint64)), cal.C4_BINS,
+                                   cal.C4_TMAX, counters=True)
+    counters = cnt.as_dict()
+    ops = roofline.algorithmic_ops(p, counters, counts[0, :5])
+    steps = roofline.photon_steps(counters)
     nc = 4_000_000_000
     lr, lr_se = cal.lr_gradient(tpl, theta, T, nc, n, parts=16, dl_dh=dl_dh)
     fd, fd_se = cal.fd_gradient(tpl, theta, T, nc, n, rel_step=1e-3, parts=16, dl_dh=dl_dh)
@@ -298,11 +389,38 @@ def bench_differentiation():
         check[nm] = {"adjoint": lr[j], "adjoint_se": lr_se[j], "fd": fd[j], "fd_se": fd_se[j],
                      "rel_diff": (lr[j] - fd[j]) / abs(fd[j]) if fd[j] else 0.0,
                      "pull": (lr[j] - fd[j]) / se if se else 0.0}
+    # CPU: forward tally (reference) on the same photons, FP64 gradient (oracle
+    # port) on a bounded sample
+    from oracle.oracle import Oracle
+    lib, kind = _cpu_lib()
+    cores = int(lib.max_threads())
+    t0 = time.perf_counter()
+    lib.tally(p, n, cal.C4_BINS, cal.C4_TMAX, 1, threads=0)
+    cf = time.perf_counter() - t0
+    orc = Oracle("libm")
+    m = 1_000_000
+    t0 = time.perf_counter()
+    orc.gradient([p], [m], cal.C4_BINS, cal.C4_TMAX, dl_dh[None, :])
+    cg = time.perf_counter() - t0
+    cpu_fwd = n / cf
+    cpu_adj = m / cg
     return {"photons": n, "loss": loss, "grad": dict(zip(names, g.tolist())),
             "forward_ms": f_ms, "adjoint_ms": a_ms,
             "fwd_adj_over_fwd": (f_ms + a_ms) / f_ms,
+            "forward_photons_per_s": n / (f_ms * 1e-3),
+            "forward_photon_steps_per_s": steps / (f_ms * 1e-3),
             "fwd_adj_photons_per_s": n / ((f_ms + a_ms) * 1e-3),
-            "gradient_check_photons": nc, "gradient_vs_fd": check}
+            "roofline": roofline.roofline(ops, f_ms, "batch_kernel (forward, one set)"),
+            "roofline_fwd_adj": roofline.roofline(
+                2 * ops, f_ms + a_ms, "batch_kernel + adjoint_kernel",
+                "transport operations of both passes; the adjoint's scores are not counted"),
+            "gradient_check_photons": nc, "gradient_vs_fd": check,
+            "cpu_baseline": {"value": 1.0 / (1.0 / cpu_fwd + 1.0 / cpu_adj), "unit": "photons/s",
+                             "cores": cores, "kind": f"{kind} forward + port FP64 gradient",
+                             "forward_photons_per_s": cpu_fwd, "gradient_photons_per_s": cpu_adj,
+                             "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                             "sample": f"forward: photon ids 0..{n - 1}; gradient: ids "
+                                       f"0..{m - 1}"}}
 
 
 def bench_gradient_checks(photons=4_000_000_000):
@@ -385,15 +503,42 @@ def bench_calibration():
     """Batched differentiable calibration: 256 parameter sets x 1e6 photons,
     long-histories geometry, (abs_length, R, n_det) fitted by 50 Adam
     iterations to targets from hidden parameters under a held-out seed; each
-    iteration one captured graph (forward, chi-square weights, adjoint)."""
+    iteration one captured graph (forward, chi-square weights, adjoint). The
+    roofline is that of a forward batch launch at the fitted parameters, from
+    its branch counters."""
+    import numpy as np
+
+    from gpu_optical_photons_amd import adjoint, roofline
     from gpu_optical_photons_amd import calibrate as cal
+    from gpu_optical_photons_amd.batch import EventBatch, run_gpu_batch
     r = cal.calibrate(n_sets=256, photons=1_000_000, iters=50)
-    return {k: r[k] for k in ("n_sets", "photons_per_set", "iterations", "median_rel_error",
-                              "p90_rel_error", "graph_ms", "forward_only_ms", "wall_s",
-                              "forward_photons_per_s", "fwd_adj_photons_per_s",
-                              "fwd_adj_over_fwd", "forward_tuning")} | {
+    tpl = cal.c5_template(r["seed"])
+    batch = EventBatch([adjoint.with_theta(tpl, q) for q in cal._theta_rows(r["fitted"], 256)],
+                       np.full(256, 1_000_000, np.uint64))
+    tu = r["forward_tuning"]
+    run_gpu_batch(batch, r["bins"], r["tmax"], tuning=tu)
+    f_ms = float(np.median([run_gpu_batch(batch, r["bins"], r["tmax"], tuning=tu)[1].kernel_ms
+                            for _ in range(3)]))
+    counts, _, cnt = run_gpu_batch(batch, r["bins"], r["tmax"], tuning=tu, counters=True)
+    counters = cnt.as_dict()
+    fates = counts[:, :5].sum(axis=0, dtype=np.uint64)
+    ops = roofline.algorithmic_ops(None, counters, fates, emitted_volume=batch.total)
+    kernel = "batch_ring_kernel" if (tu or {}).get("stream_in_lds") else "batch_kernel"
+    out = {k: r[k] for k in ("n_sets", "photons_per_set", "iterations", "median_rel_error",
+                             "p90_rel_error", "graph_ms", "forward_only_ms", "wall_s",
+                             "forward_photons_per_s", "fwd_adj_photons_per_s",
+                             "fwd_adj_over_fwd", "forward_tuning")}
+    out.update({
         "final_median_loss": r["losses"][-1], "first_median_loss": r["losses"][0],
-        "diagnostics": cal.calibration_diagnostics(r)}
+        "forward_at_fit_ms": f_ms,
+        "forward_photon_steps_per_s": roofline.photon_steps(counters) / (f_ms * 1e-3),
+        "roofline": roofline.roofline(ops, f_ms, kernel + " (forward, 256 sets)"),
+        "roofline_graph": roofline.roofline(
+            2 * ops * r["iterations"], r["graph_ms"], "calibration step graph",
+            "forward + adjoint transport operations at the fitted parameters x 50; the "
+            "adjoint's scores are not counted"),
+        "diagnostics": cal.calibration_diagnostics(r)})
+    return out
 
 
 def load_traffic():
@@ -596,6 +741,10 @@ def main():
         line["batched_events"] = bench_batched()
         line["batched_vs_plain"] = (line["batched_events"]["plain_ms"]
                                     / line["batched_events"]["wavefront_ms"])
+        # the same photons through both kernels at one large launch: what the
+        # wavefront kernel itself gains over the history kernel, apart from
+        # the launch count of the per-event baseline
+        line["wavefront_over_history_same_photons"] = htm.kernel_ms / transport_avg
         line["gensteps"] = bench_gensteps()
         line["differentiation"] = bench_differentiation()
         line["calibration"] = bench_calibration()

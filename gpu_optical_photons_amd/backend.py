@@ -28,7 +28,8 @@ RECORD_DTYPES = (np.uint8, np.uint32, np.uint32, np.float32, np.float32, np.floa
 OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_NO_DEVICE = 0, 1, 2, 3
 
 CNT_NAMES = ("useful_lane_steps", "issued_lane_steps", "absorb", "scatter", "readout", "polished",
-             "black", "reflector", "lambert_reflect", "emitted", "detected")
+             "black", "reflector", "lambert_reflect", "emitted", "detected", "abs_on_steps",
+             "scat_on_steps")
 
 
 class RunTiming(ctypes.Structure):

@@ -39,6 +39,8 @@ enum : int {
   kCntLambert = 8,
   kCntEmitted = 9,
   kCntDetected = 10,
+  kCntAbsOn = 11,   // photon-steps that sampled an absorption distance
+  kCntScatOn = 12,  // photon-steps that sampled a scattering distance
   kCntSlots = 16
 };
 

@@ -496,6 +496,7 @@ __device__ __forceinline__ void fresnel_score(float n1f, float n2f, float cif, b
 // Per-step classification, exposed so kernels can count branch events.
 struct StepInfo {
   bool absorb, scatter, readout, polished, black, refl, survived, lambert_reflect;
+  bool abs_on, scat_on;  // the step sampled an absorption / scattering distance
 };
 
 // Emission (transport.hpp:55-67): [x, y, z if volume], cos, phi, time.
@@ -820,6 +821,8 @@ __device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, u
   info.refl = refl;
   info.survived = survived;
   info.lambert_reflect = lambert_reflect;
+  info.abs_on = P.abs_length != kInf;
+  info.scat_on = P.scat_length != kInf;
   return alive;
 }
 

@@ -75,6 +75,8 @@ struct WaveCounters {
     v[kCntBlack] += popc(__ballot(s.black));
     v[kCntReflector] += popc(__ballot(s.refl));
     v[kCntLambert] += popc(__ballot(s.lambert_reflect));
+    v[kCntAbsOn] += popc(__ballot(s.abs_on));
+    v[kCntScatOn] += popc(__ballot(s.scat_on));
   }
   __device__ __forceinline__ void flush(unsigned long long* dst) const {
     if (kOn && lane_id() == 0) {
@@ -734,6 +736,8 @@ __global__ __launch_bounds__(1024) void wavefront_phase_kernel(DevParams P0, Phi
       info.polished = side && surface == kPolished;
       info.black = side && surface == kBlack;
       info.refl = side && surface >= kSpecular;
+      info.abs_on = P.abs_length != kInf;
+      info.scat_on = P.scat_length != kInf;
       uint32_t used = 2u;
       if (a.absorb) {
         finished = true;

@@ -69,15 +69,29 @@ def step_common(abs_length: float, scat_length: float) -> float:
     return 2 * DRAW + e_abs + e_sca + BOX + DECIDE
 
 
-def algorithmic_ops(params, counters: dict, fate_counts) -> float:
+def algorithmic_ops(params, counters: dict, fate_counts, emitted_volume=None) -> float:
     """Total operations of the reference algorithm for a run, from the
-    kernel's branch-event counters and the run's fate counts."""
+    kernel's branch-event counters and the run's fate counts. For a batch of
+    events with different parameters pass params=None: the counters'
+    abs_on_steps / scat_on_steps give the steps that sampled each distance,
+    and emitted_volume the photons of volume-source events (the rest are
+    point-source emissions)."""
     c = counters
     steps = c["useful_lane_steps"]
     detected = int(fate_counts[0])
     escaped = int(fate_counts[3])
     absorbed_surface = int(fate_counts[2])
-    ops = steps * step_common(params.abs_length, params.scat_length)
+    if params is not None:
+        abs_on = 0 if math.isinf(params.abs_length) else steps
+        scat_on = 0 if math.isinf(params.scat_length) else steps
+        if emitted_volume is None:
+            emitted_volume = c["emitted"] if params.source == 1 else 0
+    else:
+        abs_on, scat_on = c["abs_on_steps"], c["scat_on_steps"]
+        emitted_volume = emitted_volume or 0
+    ops = steps * (2 * DRAW + BOX + DECIDE)
+    ops += abs_on * EXPD + (steps - abs_on) * 1.0
+    ops += scat_on * EXPD + (steps - scat_on) * 1.0
     ops += c["absorb"] * ADVANCE
     ops += c["scatter"] * (ADVANCE + 2 * DRAW + RAYLEIGH_COS + ROTATE + 1.0)
     hits = c["readout"] + c["polished"] + c["black"] + c["reflector"]
@@ -91,9 +105,21 @@ def algorithmic_ops(params, counters: dict, fate_counts) -> float:
     lam = c["lambert_reflect"]
     ops += (refl_survived - lam) * REFLECT
     ops += lam * (2 * DRAW + 1.0 + 3.0 + ROTATE)
-    emit = EMIT_VOLUME if params.source == 1 else EMIT_POINT
-    ops += c["emitted"] * emit
+    ops += emitted_volume * EMIT_VOLUME + (c["emitted"] - emitted_volume) * EMIT_POINT
     return float(ops)
+
+
+def roofline(ops: float, kernel_ms: float, kernel: str, note: str = "") -> dict:
+    """The roofline block of a launch: algorithmic operations over its time,
+    against the FP32 vector peak (and the non-packed lane-op ceiling)."""
+    achieved = ops / (kernel_ms * 1e-3) / 1e12
+    out = {"bound": "valu", "achieved": achieved, "peak": FP32_VECTOR_PEAK_TFLOPS,
+           "unit": "TFLOP/s", "frac": achieved / FP32_VECTOR_PEAK_TFLOPS, "kernel": kernel,
+           "kernel_ms": kernel_ms, "algorithmic_ops_per_launch": ops,
+           "frac_of_valu_issue_peak": achieved / VALU_ISSUE_PEAK_TOPS}
+    if note:
+        out["note"] = note
+    return out
 
 
 def photon_steps(counters: dict) -> int:

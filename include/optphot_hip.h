@@ -99,6 +99,8 @@ enum {
   OPTPHOT_CNT_LAMBERT_REFLECT = 8,
   OPTPHOT_CNT_EMITTED = 9,
   OPTPHOT_CNT_DETECTED = 10,
+  OPTPHOT_CNT_ABS_ON = 11,  /* photon-steps with finite abs_length (distance sampled) */
+  OPTPHOT_CNT_SCAT_ON = 12, /* photon-steps with finite scat_length */
   OPTPHOT_CNT_SLOTS = 16
 };
 typedef struct optphot_step_counters {
