@@ -295,7 +295,7 @@ def bench_reference_workloads():
     return out
 
 
-def bench_batched():
+def bench_batched(cpu=True):
     """Batched events: 4096 events (box ~U[10,100] mm per axis, surface model
     uniform over polished / black / specular / Lambertian, n_det ~U[1.4,1.6],
     abs ~U[300,3000] mm, scat inf or ~U[100,1000] mm, point source uniform,
@@ -325,14 +325,20 @@ def bench_batched():
     ops = roofline.algorithmic_ops(None, counters, fates, emitted_volume=volume)
     steps = roofline.photon_steps(counters)
     kernel = "batch_ring_kernel" if (tu or {}).get("stream_in_lds") else "batch_kernel"
-    # CPU: the reference's tally of every event, photon ids at the event's offset
-    lib, kind = _cpu_lib()
-    cores = int(lib.max_threads())
-    offs = b.offsets
-    t0 = time.perf_counter()
-    for e in range(b.n_events):
-        lib.tally(b.params[e], int(b.photons[e]), 128, 50.0, 1, first_id=int(offs[e]), threads=0)
-    dt = time.perf_counter() - t0
+    cpu_baseline = None
+    if cpu:  # the reference's tally of every event, photon ids at the event's offset
+        lib, kind = _cpu_lib()
+        cores = int(lib.max_threads())
+        offs = b.offsets
+        t0 = time.perf_counter()
+        for e in range(b.n_events):
+            lib.tally(b.params[e], int(b.photons[e]), 128, 50.0, 1, first_id=int(offs[e]),
+                      threads=0)
+        dt = time.perf_counter() - t0
+        cpu_baseline = {"value": b.total / dt, "unit": "photons/s", "cores": cores, "kind": kind,
+                        "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                        "sample": f"all {b.n_events} events, one tally call per event "
+                                  f"({dt:.1f} s wall)"}
     return {"events": b.n_events, "photons": b.total, "tuning": tu, "wavefront_ms": w_ms,
             "wavefront_ms_runs": ms, "plain_ms": tp.kernel_ms,
             "wavefront_photons_per_s": b.total / (w_ms * 1e-3),
@@ -342,10 +348,7 @@ def bench_batched():
             "simt_efficiency": counters["useful_lane_steps"] / counters["issued_lane_steps"],
             "roofline": roofline.roofline(ops, w_ms, kernel),
             "identical": bool(np.array_equal(w, pl) and np.array_equal(w, wc)),
-            "cpu_baseline": {"value": b.total / dt, "unit": "photons/s", "cores": cores,
-                             "kind": kind, "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
-                             "sample": f"all {b.n_events} events, one tally call per event "
-                                       f"({dt:.1f} s wall)"}}
+            "cpu_baseline": cpu_baseline}
 
 
 def bench_differentiation():
@@ -618,7 +621,7 @@ def main():
         print(json.dumps(out))
         return
     if args.batched_only:
-        print(json.dumps({"batched_events": bench_batched()}))
+        print(json.dumps({"batched_events": bench_batched(cpu=not args.no_cpu)}))
         return
     if args.gensteps_only:
         print(json.dumps({"gensteps": bench_gensteps()}))

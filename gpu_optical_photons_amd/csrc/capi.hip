@@ -1323,12 +1323,13 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
       c.drop_graphs();  // buffers moved: captured graphs are stale
     }
     const bool ring = tu.stream_in_lds != 0 && method == OPTPHOT_BATCH_WAVEFRONT;
-    if (ring && batch_ring_lds_bytes(batch_lds_bytes(n_events, words), tu.block_size) > c.max_lds)
+    if (ring && batch_ring_lds_bytes(batch_tally_lds_bytes(n_events, words, tu.block_size),
+                                     tu.block_size) > c.max_lds)
       throw InvalidArgument("batch: event tallies plus LDS streams exceed the LDS of a block; use "
                             "a smaller block_size");
     WavefrontShape s = [&] {
       const int max_per_cu = std::max(
-          1, batch_blocks_per_cu(tu.block_size, batch_lds_bytes(n_events, words), ring));
+          1, batch_blocks_per_cu(tu.block_size, n_events, words, ring));
       const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
       const uint64_t needed = (total + tu.block_size - 1) / tu.block_size;
       WavefrontShape w{};
@@ -1374,6 +1375,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_batch(
     lo.evt.offsets = c.batch_offsets.as<unsigned long long>();
     lo.evt.slots = c.batch_slots.as<unsigned long long>();
     lo.evt.n_events = n_events;
+    lo.evt.wave_rows = batch_lds_bytes(n_events, words) == 0 &&
+                       batch_wave_row_bytes(words, s.block) != 0;
     const PhiloxKey key = key_of(params[0]);
 
     const uint64_t n_slots = slots.size();
@@ -1647,11 +1650,12 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
     WavefrontShape sf{};
     {
       const bool ring = tf.stream_in_lds != 0;
-      if (ring && batch_ring_lds_bytes(batch_lds_bytes(n_sets, words), tf.block_size) > c.max_lds)
+      if (ring && batch_ring_lds_bytes(batch_tally_lds_bytes(n_sets, words, tf.block_size),
+                                       tf.block_size) > c.max_lds)
         throw InvalidArgument("chi2 step: event tallies plus LDS streams exceed the LDS of a "
                               "block; use a smaller block_size");
       const int max_per_cu = std::max(
-          1, batch_blocks_per_cu(tf.block_size, batch_lds_bytes(n_sets, words), ring));
+          1, batch_blocks_per_cu(tf.block_size, n_sets, words, ring));
       const int per_cu =
           tf.blocks_per_cu > 0 ? std::min(tf.blocks_per_cu, max_per_cu) : max_per_cu;
       const uint64_t needed = (total + tf.block_size - 1) / tf.block_size;
@@ -1682,6 +1686,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_chi2_step(
     lo.evt.offsets = c.batch_offsets.as<unsigned long long>();
     lo.evt.slots = c.batch_slots.as<unsigned long long>();
     lo.evt.n_events = n_sets;
+    lo.evt.wave_rows = batch_lds_bytes(n_sets, words) == 0 &&
+                       batch_wave_row_bytes(words, sf.block) != 0;
     const PhiloxKey key = key_of(params[0]);
     const uint64_t n_slots = slots.size();
     sf.slots_per_claim = slots_per_claim(n_slots, static_cast<uint64_t>(sf.grid) * sf.block / 64u);

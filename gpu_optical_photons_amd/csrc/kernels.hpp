@@ -59,6 +59,8 @@ struct DevEventTally {
                                     // (slots[c] & 0xffffffff) of event slots[c] >> 32
   uint32_t n_events;
   uint32_t event;                   // the single event of a plain per-event launch
+  uint32_t wave_rows;               // batch launches with rows in global memory: each
+                                    // wave keeps its current event's row in LDS
 };
 
 // Output selection of a transport launch.
@@ -211,7 +213,14 @@ size_t batch_lds_bytes(uint32_t n_events, uint32_t words);
 // LDS bytes of a batch launch with ring streams: the ring region
 // (ring_words), then the event tallies (if any).
 size_t batch_ring_lds_bytes(size_t tally_lds, int block);
-int batch_blocks_per_cu(int block, size_t lds, bool ring = false);
+// Otherwise each wave of a block keeps the row of the event it is drawing
+// photons from in LDS: batch_wave_row_bytes() is that LDS for a block, or 0
+// when it exceeds kBatchLdsMax (the rows then take global atomics only).
+size_t batch_wave_row_bytes(uint32_t words, int block);
+// The tally LDS of a batch launch: the block's copy of every event row, else
+// the waves' rows (if they fit), else 0.
+size_t batch_tally_lds_bytes(uint32_t n_events, uint32_t words, int block);
+int batch_blocks_per_cu(int block, uint32_t n_events, uint32_t words, bool ring = false);
 
 // Adjoint (likelihood-ratio + pathwise) pass over the histories of a set of
 // parameter sets, recomputed from their Philox counters.

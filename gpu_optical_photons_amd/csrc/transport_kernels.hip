@@ -161,18 +161,67 @@ struct EventFates {
   }
 };
 
-// A finished photon of event e of a batch: its fate into the lane's counts
-// (kDirect: straight into the row, for rows in LDS, where atomics are cheap),
-// and, if detected, its arrival-time bin into the event's row.
+// The row of the event a wave is drawing photons from, kept in LDS by that
+// wave (batches whose event rows do not all fit in a block's LDS). A wave
+// refills its lanes from one slot (one chunk of one event) at a time, so most
+// of its finished photons belong to the event of its current slot: their
+// fates and arrival-time bins add to the wave's row with LDS atomics, and the
+// row goes to the event's global row, one atomic per nonzero word, when the
+// wave opens a slot of another event and at the end. Photons of an earlier
+// slot's event still in flight add to the global rows directly. Integer sums:
+// the rows are unchanged.
+struct WaveRow {
+  uint32_t* row;             // `words` u32 of this wave in LDS (used when `on`)
+  uint32_t e = 0xffffffffu;  // event the row holds (wave-uniform; none yet)
+  bool on = false;
+
+  // All lanes of the wave, in wave-uniform control flow.
+  __device__ __forceinline__ void clear(uint32_t words) {
+    if (!on) return;
+    for (uint32_t w = lane_id(); w < words; w += 64u) row[w] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  }
+  __device__ __forceinline__ void flush(const DevEventTally& T) {
+    if (!on || e == 0xffffffffu) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t* dst = T.counts + static_cast<size_t>(e) * T.words;
+    for (uint32_t w = lane_id(); w < T.words; w += 64u) {
+      const uint32_t v = row[w];
+      if (v) {
+        atomicAdd(dst + w, v);
+        row[w] = 0u;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  }
+  __device__ __forceinline__ void take(const DevEventTally& T, uint32_t event) {
+    if (!on || event == e) return;
+    flush(T);
+    e = event;
+  }
+};
+
+// A finished photon of event e of a batch: its fate and, if detected, its
+// arrival-time bin into the event's row. kDirect: straight into the row (rows
+// in LDS, where atomics are cheap); else into the wave's LDS row when it holds
+// event e, otherwise the fate into the lane's counts and the time bin into
+// the global row.
 template <bool kDirect>
 __device__ __forceinline__ void finish_event(const DevEventTally& T, EventFates& fates,
-                                             uint32_t e, const Photon& ph, uint32_t fate) {
+                                             WaveRow& wr, uint32_t e, const Photon& ph,
+                                             uint32_t fate) {
+  const uint32_t tb = fate == kDetected ? 5u + time_bin(ph.t, T.time_scale, T.time_bins) : 0u;
+  if (!kDirect && e == wr.e) {  // wave-scope LDS adds (only this wave uses the row)
+    __hip_atomic_fetch_add(wr.row + fate, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (tb) __hip_atomic_fetch_add(wr.row + tb, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    return;
+  }
   uint32_t* row = T.counts + static_cast<size_t>(e) * T.words;
   if (kDirect)
     atomicAdd(row + fate, 1u);
   else
     fates.add(T, e, fate);
-  if (fate == kDetected) atomicAdd(row + 5u + time_bin(ph.t, T.time_scale, T.time_bins), 1u);
+  if (tb) atomicAdd(row + tb, 1u);
 }
 
 template <bool kTally, bool kCount>
@@ -899,6 +948,10 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   WaveCounters<kCount> wc;
   wc.clear();
   EventFates fates;
+  WaveRow wr;
+  wr.on = !kLds && out.evt.wave_rows != 0u;
+  wr.row = evl + (threadIdx.x / 64u) * out.evt.words;
+  wr.clear(out.evt.words);
 
   uint32_t we = 0;                 // event of the wave's claimed range (uniform)
   uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
@@ -917,7 +970,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
     const bool all_dead = __ballot(alive) == 0;
     if (all_dead || it % steps_per_refill == 0) {
       if (pending) {
-        finish_event<kLds>(loc.evt, fates, pset, ph, fate);
+        finish_event<kLds>(loc.evt, fates, wr, pset, ph, fate);
         pending = false;
       }
       const uint64_t dead = __ballot(!alive);
@@ -948,6 +1001,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
           const uint64_t start = static_cast<uint64_t>(static_cast<uint32_t>(sl)) * chunk;
           const uint64_t ne = off[e + 1] - off[e];
           if (start >= ne) continue;
+          wr.take(loc.evt, e);
           we = e;
           q_next = start;
           q_end = (start + chunk < ne) ? start + chunk : ne;
@@ -1005,6 +1059,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
     if (finished) pending = true;
   }
   fates.flush(loc.evt);
+  wr.flush(loc.evt);
   wc.flush(out.counters);
   if (kLds) {
     __syncthreads();
@@ -1269,7 +1324,8 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
 //    and that history then runs alongside the bulk instead of after it;
 //  * within that order, chunk j of every event precedes chunk j + 1, so the
 //    waves in flight at any moment work on many different events and the
-//    per-event tally atomics spread over many rows instead of queueing on one;
+//    per-event tally atomics that remain (WaveRow) spread over many rows
+//    instead of queueing on one;
 //  * events with the same branch structure (surface model, scattering on,
 //    absorption on) are adjacent, so the consecutive slots a wave claims
 //    mostly share it: branch bodies that none of the wave's events need (the
@@ -1281,8 +1337,9 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
 //
 // With kLds (few events: n_events * words u32 fit in LDS) each block keeps
 // its own copy of all event tallies in LDS and adds it to the global rows
-// once at the end; otherwise finished photons add to the global rows
-// directly, spread over many events by the interleaved queue.
+// once at the end; otherwise each wave keeps the row of its current slot's
+// event in LDS (WaveRow, when out.evt.wave_rows) and photons of other events
+// add to the global rows directly.
 template <bool kCount, bool kLds>
 __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t first_id,
                                                      uint32_t steps_per_refill, uint32_t chunk,
@@ -1304,6 +1361,10 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
   WaveCounters<kCount> wc;
   wc.clear();
   EventFates fates;
+  WaveRow wr;
+  wr.on = !kLds && out.evt.wave_rows != 0u;
+  wr.row = lds + (threadIdx.x / 64u) * out.evt.words;
+  wr.clear(out.evt.words);
 
   uint32_t we = 0;                 // event of the wave's claimed range (uniform)
   uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
@@ -1347,6 +1408,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
         const uint64_t start = static_cast<uint64_t>(static_cast<uint32_t>(sl)) * chunk;
         const uint64_t ne = off[e + 1] - off[e];
         if (start >= ne) continue;
+        wr.take(loc.evt, e);
         we = e;
         q_next = start;
         q_end = (start + chunk < ne) ? start + chunk : ne;
@@ -1370,7 +1432,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
         fresh = true;
       }
       wc.add(kCntEmitted, t1 + t2);
-      if (fresh && !alive) finish_event<kLds>(loc.evt, fates, pset, ph, fate);
+      if (fresh && !alive) finish_event<kLds>(loc.evt, fates, wr, pset, ph, fate);
     }
     if (__ballot(alive) == 0) {
       if (q_done) break;
@@ -1393,10 +1455,11 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
       }
       wc.add_step(act, info);
       if (kCount) wc.add(kCntDetected, popc(__ballot(finished && fate == kDetected)));
-      if (finished) finish_event<kLds>(loc.evt, fates, pset, ph, fate);
+      if (finished) finish_event<kLds>(loc.evt, fates, wr, pset, ph, fate);
     }
   }
   fates.flush(loc.evt);
+  wr.flush(loc.evt);
   wc.flush(out.counters);
   if (kLds) {
     __syncthreads();
@@ -1890,8 +1953,12 @@ void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
                             const WavefrontShape& s, unsigned long long* queue,
                             const LaunchOut& out, hipStream_t stream) {
   const size_t lds = batch_lds_bytes(out.evt.n_events, out.evt.words);
+  // without the block copies of all rows: the waves' rows, if the host chose them
+  const size_t rows = !lds && out.evt.wave_rows ? batch_wave_row_bytes(out.evt.words, s.block) : 0;
+  if (!lds && out.evt.wave_rows && rows == 0)
+    throw std::invalid_argument("launch_batch_wavefront: wave rows exceed the LDS budget");
   if (s.stream_in_lds) {
-    const size_t bytes = batch_ring_lds_bytes(lds, s.block);
+    const size_t bytes = batch_ring_lds_bytes(lds ? lds : rows, s.block);
     if (lds && out.counters)
       hipLaunchKernelGGL((batch_ring_kernel<true, true>), dim3(s.grid), dim3(s.block), bytes,
                          stream, key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
@@ -1919,13 +1986,13 @@ void launch_batch_wavefront(PhiloxKey key, uint64_t first_id, uint64_t n_slots,
                        key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
                        s.slots_per_claim, queue, out);
   else if (out.counters)
-    hipLaunchKernelGGL((batch_kernel<true, false>), dim3(s.grid), dim3(s.block), 0, stream, key,
-                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, s.slots_per_claim,
-                       queue, out);
+    hipLaunchKernelGGL((batch_kernel<true, false>), dim3(s.grid), dim3(s.block), rows, stream,
+                       key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
+                       s.slots_per_claim, queue, out);
   else
-    hipLaunchKernelGGL((batch_kernel<false, false>), dim3(s.grid), dim3(s.block), 0, stream, key,
-                       first_id, s.steps_per_refill, s.queue_chunk, n_slots, s.slots_per_claim,
-                       queue, out);
+    hipLaunchKernelGGL((batch_kernel<false, false>), dim3(s.grid), dim3(s.block), rows, stream,
+                       key, first_id, s.steps_per_refill, s.queue_chunk, n_slots,
+                       s.slots_per_claim, queue, out);
 }
 
 void launch_history_event(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
@@ -2069,6 +2136,16 @@ size_t batch_ring_lds_bytes(size_t tally_lds, int block) {
   return ring_words(block) * sizeof(uint32_t) + tally_lds;
 }
 
+size_t batch_wave_row_bytes(uint32_t words, int block) {
+  const size_t b = static_cast<size_t>(block / 64) * words * sizeof(uint32_t);
+  return b <= kBatchLdsMax ? b : 0;
+}
+
+size_t batch_tally_lds_bytes(uint32_t n_events, uint32_t words, int block) {
+  const size_t all = batch_lds_bytes(n_events, words);
+  return all ? all : batch_wave_row_bytes(words, block);
+}
+
 void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,
                      const WavefrontShape& shape, const GenstepArgs& g,
                      unsigned long long* queue, const LaunchOut& out, hipStream_t stream) {
@@ -2091,14 +2168,16 @@ int genstep_blocks_per_cu(bool records, int block, size_t lds, bool ring) {
                  : blocks_per_cu(genstep_kernel<false, false, kLambertian>, block, lds);
 }
 
-int batch_blocks_per_cu(int block, size_t lds, bool ring) {
+int batch_blocks_per_cu(int block, uint32_t n_events, uint32_t words, bool ring) {
+  const bool all = batch_lds_bytes(n_events, words) != 0;
+  const size_t lds = batch_tally_lds_bytes(n_events, words, block);
   if (ring)
-    return lds ? blocks_per_cu(batch_ring_kernel<false, true>, block,
+    return all ? blocks_per_cu(batch_ring_kernel<false, true>, block,
                                batch_ring_lds_bytes(lds, block))
                : blocks_per_cu(batch_ring_kernel<false, false>, block,
-                               batch_ring_lds_bytes(0, block));
-  return lds ? blocks_per_cu(batch_kernel<false, true>, block, lds)
-             : blocks_per_cu(batch_kernel<false, false>, block, 0);
+                               batch_ring_lds_bytes(lds, block));
+  return all ? blocks_per_cu(batch_kernel<false, true>, block, lds)
+             : blocks_per_cu(batch_kernel<false, false>, block, lds);
 }
 
 }  // namespace ophip

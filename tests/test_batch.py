@@ -111,3 +111,18 @@ def test_batch_first_id_and_tuning_invariance(backend):
         assert np.array_equal(w, ref)
     other, _, _ = run_gpu_batch(b, TB, TMAX, first_id=0)
     assert not np.array_equal(other, ref)
+
+
+@pytest.mark.gpu
+def test_batch_wide_time_rows(backend):
+    """4096-bin time rows: at block size 64 each wave's LDS row (16 KB) fits the
+    LDS budget, at 1024 the 16 rows of a block do not and every photon adds to
+    the global rows; both equal the per-event launches."""
+    from gpu_optical_photons_amd.batch import PLAIN, WAVEFRONT, make_events, run_gpu_batch
+    b = make_events(300, seed=5, photons_per_mev=2000)
+    plain, _, _ = run_gpu_batch(b, 4096, TMAX, method=PLAIN)
+    for tu in (dict(block_size=64, steps_per_refill=2, queue_chunk=128),
+               dict(block_size=1024, steps_per_refill=2, queue_chunk=128),
+               dict(block_size=64, steps_per_refill=2, queue_chunk=128, stream_in_lds=1)):
+        w, _, _ = run_gpu_batch(b, 4096, TMAX, method=WAVEFRONT, tuning=tu)
+        assert np.array_equal(w, plain), tu
