@@ -770,6 +770,18 @@ ORACLE_EXPORT uint32_t oracle_trace(const oracle_params* p, uint64_t id, uint32_
   return step;
 }
 
+/* The emitted state of photon `id` (position, direction, time: 7 floats),
+   i.e. the state before the first step of oracle_trace (test
+   instrumentation: the first step's flight starts here). */
+ORACLE_EXPORT void oracle_emitted(const oracle_params* p, uint64_t id, float* state) {
+  stream_t rng;
+  stream_init(&rng, p->seed, id, 0u);
+  const pstate s = start_photon(p, &rng);
+  state[0] = s.pos.x, state[1] = s.pos.y, state[2] = s.pos.z;
+  state[3] = s.dir.x, state[4] = s.dir.y, state[5] = s.dir.z;
+  state[6] = s.t;
+}
+
 ORACLE_EXPORT void oracle_run_records(const oracle_params* p, uint64_t n, uint64_t first_id,
                                       int threads, uint8_t* fate, uint32_t* n_boundary,
                                       uint32_t* n_scatter, float* t, float* x, float* y, float* z,

@@ -134,6 +134,7 @@ class Oracle(_Lib):
                                    ctypes.c_void_p, ctypes.c_int])
         self._fn("trace", ctypes.c_uint32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
                                             _u8p, _f32p, ctypes.c_void_p])
+        self._fn("emitted", None, [ctypes.c_void_p, ctypes.c_uint64, _f32p])
         self._fn("run_gensteps", None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                         ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -154,6 +155,13 @@ class Oracle(_Lib):
         self.run_gensteps(ctypes.byref(params), g.ctypes.data, len(g), first_id,
                           ctypes.byref(cfg), flat.ctypes.data, *ptrs, threads)
         return split_tally(flat, time_bins, xy_bins), r
+
+    def emitted_state(self, params, photon_id):
+        """Position, direction and time (7 floats) of photon `photon_id` as
+        emitted: the state before the first row of step_trace."""
+        out = np.zeros(7, np.float32)
+        self.emitted(ctypes.byref(params), photon_id, out)
+        return out
 
     def step_trace(self, params, photon_id, cap=1 << 14, with_state=False):
         """(codes u8 [steps], margins f32 [steps]) of one photon: per step the

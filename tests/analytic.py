@@ -72,6 +72,12 @@ def binomial_pull(k, n, p):
     return (k - n * p) / math.sqrt(n * p * (1.0 - p))
 
 
+def binomial_p(k, n, p):
+    """Two-sided p-value of k successes in n trials of probability p (normal
+    approximation; n p (1 - p) is in the thousands in every use)."""
+    return math.erfc(abs(binomial_pull(k, n, p)) / math.sqrt(2.0))
+
+
 def face_integral(p, face, integrand, panels=24, order=16):
     """(1/4pi) Int over `face` of integrand(r, cos_i) dOmega for a point source,
     dOmega = h dA / r^3 and cos_i = h / r (the angle to the face normal), by the

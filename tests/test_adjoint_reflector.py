@@ -75,6 +75,6 @@ def test_reflector_score_against_mirror_box(oracle_portable, R):
     assert abs(g[2] * R - dP) > 10.0 * se[2]
     # the forward estimate of the same photons: binomial agreement with <R^k>
     t = oracle_portable.tally(p, n, tb, tmax, 1)
-    assert abs(analytic.binomial_pull(int(t["fate_counts"][0]), n, P)) < 4.0
+    assert analytic.binomial_p(int(t["fate_counts"][0]), n, P) > 0.01
     # no other parameter enters this box
     assert g[0] == 0.0 and g[1] == 0.0

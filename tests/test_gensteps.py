@@ -63,10 +63,10 @@ def test_emission_along_segment(oracle_portable):
     pos = np.stack([r["x"], r["y"], r["z"]], axis=1).astype(np.float64)
     u = (pos - a) @ (b - a) / ((b - a) @ (b - a))
     assert np.abs(np.linalg.norm((pos - a) - np.outer(u, b - a), axis=1)).max() < 1e-3
-    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    assert stats.kstest(u, "uniform").pvalue > 0.01
     delay = r["t"].astype(np.float64) - (1.0 + u * 0.5) - r["path"] * 1.63 / 299.792458
     assert abs(delay.mean() - 3.0) < 5 * 3.0 / np.sqrt(n)
-    assert stats.kstest(delay, "expon", args=(0.0, 3.0)).pvalue > 1e-3
+    assert stats.kstest(delay, "expon", args=(0.0, 3.0)).pvalue > 0.01
 
 
 def test_global_index_and_first_id(oracle_portable):
@@ -119,11 +119,11 @@ def test_cherenkov_cone_geometry(oracle_portable):
     x, y, path = (r[k].astype(np.float64) for k in ("x", "y", "path"))
     sin_t = math.sqrt(1.0 - cone * cone)
     assert np.abs(np.hypot(x, y) / path - sin_t).max() < 1e-5
-    assert stats.kstest((np.arctan2(y, x) / (2 * np.pi)) % 1.0, "uniform").pvalue > 1e-3
+    assert stats.kstest((np.arctan2(y, x) / (2 * np.pi)) % 1.0, "uniform").pvalue > 0.01
     z_emit = 50.0 - path * cone
     u = (z_emit + 40.0) / 2.0
     assert u.min() > -1e-4 and u.max() < 1.0 + 1e-4
-    assert stats.kstest(u, "uniform").pvalue > 1e-3
+    assert stats.kstest(u, "uniform").pvalue > 0.01
     t_emit = r["t"].astype(np.float64) - path * 1.6 / 299.792458
     assert np.abs(t_emit - (1.0 + 0.2 * u)).max() < 1e-4  # no emission delay
 
