@@ -124,6 +124,24 @@ def bench_plain_small():
                                   dev_w[f].cpu().numpy().view(np.uint8))
                for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"))
     steps = int(rec.steps().sum())
+    # the library's automatic kernel choice and the autotuned records shape
+    # (tuning.json), into device arrays as above; medians of 5 after a warm-up
+    from gpu_optical_photons_amd import autotune
+
+    def _records_ms(tuning):
+        backend.run_gpu_records_tuned(p, n, tuning, device="cuda:0")
+        runs = [backend.run_gpu_records_tuned(p, n, tuning, device="cuda:0") for _ in range(5)]
+        same_as_history = all(
+            np.array_equal(r[0][f].cpu().numpy().view(np.uint8), dev[f].cpu().numpy().view(np.uint8))
+            for r in runs[:1] for f in ("fate", "n_boundary", "n_scatter", "t", "x", "y", "z", "path"))
+        return (float(np.median([r[1].kernel_ms for r in runs])), runs[0][3].as_dict(),
+                runs[0][2], same_as_history)
+
+    auto_ms, auto_choice, auto_st, auto_same = _records_ms(None)
+    tuned = autotune.records_choice_for(p, n)
+    tuned_ms, tuned_choice, tuned_st, tuned_same = (_records_ms(tuned) if tuned
+                                                    else (None, None, None, True))
+    same = same and auto_same and tuned_same
     # branch counts of the same photons (history tally, counting variant)
     c = TallyConfig()
     tally, _, cnt = backend.run_gpu_tally(p, c, n, counters=True)
@@ -163,6 +181,15 @@ def bench_plain_small():
             "roofline": roofline.roofline(ops, tm.kernel_ms, "history_kernel (records)"),
             "roofline_wavefront": roofline.roofline(ops, tme.kernel_ms,
                                                     "wavefront_kernel (records)"),
+            "records_auto": {"chosen": auto_choice, "kernel_ms": auto_ms,
+                             "photons_per_s": n / (auto_ms * 1e-3),
+                             "simt_efficiency": (auto_st.simt_efficiency()
+                                                 if auto_st.issued_lane_steps else None)},
+            "records_autotuned": None if tuned is None else {
+                "tuning": tuned_choice, "kernel_ms": tuned_ms,
+                "photons_per_s": n / (tuned_ms * 1e-3),
+                "simt_efficiency": (tuned_st.simt_efficiency()
+                                    if tuned_st.issued_lane_steps else None)},
             "records_write_GBps": n * 29 / (tm.kernel_ms * 1e-3) / 1e9,
             "records_to_host_total_ms": total_ms, "records_to_host_d2h_ms": d2h_ms,
             "records_to_host_photons_per_s": n / (total_ms * 1e-3),

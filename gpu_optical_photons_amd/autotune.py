@@ -1,4 +1,5 @@
-"""Autotuner for the launch shape of the persistent wavefront kernels.
+"""Autotuner for the launch shape of the persistent wavefront kernels and of
+records runs.
 
 Tunes block size, the refill policy -- steps between refills ("wavefront
 length"), or the successor kernel's emission threshold (lanes lacking a
@@ -6,7 +7,8 @@ successor before a wave emits) --, the queue chunk a wave claims per atomic,
 and persistent blocks per CU, by timing the real kernel on a sample of the
 workload; for batches of events (one persistent launch over all events),
 block size, steps between refills, queue chunk and the streams' place
-(registers or LDS rings), by timing the batch itself. Results never depend on the choice
+(registers or LDS rings), by timing the batch itself; for records, the kernel
+(history or wavefront) and its shape. Results never depend on the choice
 (integer tallies, per-photon streams; checked by the sweep itself and by the
 GPU tests), only the speed does. Choices persist in tuning.json next to this
 file, keyed by a workload signature (kernel, surface model, source, which bulk
@@ -195,6 +197,76 @@ def sweep_batch(batch, time_bins: int, time_max: float, blocks=BATCH_BLOCKS,
     return entry
 
 
+RECORDS_HISTORY_BLOCKS = (64, 128, 256, 512, 1024)
+RECORDS_BLOCKS = (128, 256, 512)
+RECORDS_STEPS = (1, 2, 4, 8)
+RECORDS_CHUNKS = (64, 256, 1024)
+RECORDS_BLOCKS_PER_CU = (0, 2)
+
+
+def records_signature(p, n: int) -> str:
+    """Workload class of a records run: the tally signature's physics classes
+    and the decade of n (how many photons each lane of a persistent grid
+    gets decides between the kernels)."""
+    return (f"records|surface={int(p.surface)}|source={int(p.source)}"
+            f"|abs={'on' if not math.isinf(p.abs_length) else 'off'}"
+            f"|scat={'on' if not math.isinf(p.scat_length) else 'off'}"
+            f"|n=1e{int(math.floor(math.log10(max(n, 1))))}")
+
+
+def records_choice_for(p, n: int):
+    """The stored records shape (an optphot_records_tuning dict) for this
+    class, or None (the library's automatic choice)."""
+    entry = _load().get(records_signature(p, n))
+    return dict(entry["tuning"]) if entry else None
+
+
+def sweep_records(p, n: int, reps: int = 3, save: bool = True, log=print) -> dict:
+    """Times the history kernel at every block size and the wavefront kernel
+    over block size, steps between refills, queue chunk and blocks per CU,
+    on n photons written to device memory; keeps the fastest. Every
+    candidate must write the same records bit for bit."""
+    import numpy as np
+
+    from . import backend
+
+    candidates = [{"kernel": backend.RECORDS_HISTORY, "block_size": b, "steps_per_refill": 0,
+                   "queue_chunk": 0, "blocks_per_cu": 0} for b in RECORDS_HISTORY_BLOCKS]
+    candidates += [{"kernel": backend.RECORDS_WAVEFRONT, "block_size": b, "steps_per_refill": k,
+                    "queue_chunk": ch, "blocks_per_cu": bpc}
+                   for b, k, ch, bpc in itertools.product(RECORDS_BLOCKS, RECORDS_STEPS,
+                                                          RECORDS_CHUNKS, RECORDS_BLOCKS_PER_CU)]
+    ref = None
+    results = []
+    for tu in candidates:
+        ms = []
+        for _ in range(reps + 1):
+            rec, tm, _, _ = backend.run_gpu_records_tuned(p, n, tu, device="cuda:0")
+            ms.append(tm.kernel_ms)
+        best = float(np.median(ms[1:]))
+        raw = np.concatenate([rec[f].cpu().numpy().view(np.uint8).ravel()
+                              for f in backend.RECORD_FIELDS])
+        if ref is None:
+            ref = raw
+        elif not np.array_equal(ref, raw):
+            raise AssertionError(f"records differ for {tu}: the kernels are not "
+                                 "schedule-independent")
+        results.append((best, tu))
+        log(f"  records {tu}: {best:8.3f} ms")
+    results.sort(key=lambda r: r[0])
+    best_ms, best_tu = results[0]
+    hist = min(ms for ms, tu in results if tu["kernel"] == backend.RECORDS_HISTORY)
+    entry = {"tuning": best_tu, "kernel_ms": best_ms, "photons": n,
+             "photons_per_s": n / (best_ms * 1e-3), "best_history_ms": hist,
+             "swept": len(results), "when": time.strftime("%Y-%m-%d %H:%M:%S")}
+    if save:
+        cache = _load()
+        cache[records_signature(p, n)] = entry
+        with open(CACHE, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return entry
+
+
 def main():
     import sys
     sys.path.insert(0, os.path.dirname(HERE))
@@ -207,6 +279,11 @@ def main():
                        ("plain small", c1, TallyConfig())):
         print(f"sweep: {name}", flush=True)
         e = sweep("wavefront_tally", p, c, 20_000_000, log=lambda s: print(s, flush=True))
+        print(f"best: {e['tuning']} {e['kernel_ms']:.3f} ms", flush=True)
+    for name, p, n in (("records, plain small", c1, 1_000_000),
+                       ("records, long histories", c2, 10_000_000)):
+        print(f"sweep: {name}", flush=True)
+        e = sweep_records(p, n, log=lambda s: print(s, flush=True))
         print(f"best: {e['tuning']} {e['kernel_ms']:.3f} ms", flush=True)
     from . import calibrate
     from .batch import make_events

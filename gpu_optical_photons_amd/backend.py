@@ -65,6 +65,20 @@ class WavefrontTuning(ctypes.Structure):
                 ("sorted_phases", ctypes.c_uint32)]
 
 
+RECORDS_AUTO, RECORDS_HISTORY, RECORDS_WAVEFRONT = 0, 1, 2
+
+
+class RecordsTuning(ctypes.Structure):
+    """optphot_records_tuning: the kernel (RECORDS_AUTO / _HISTORY /
+    _WAVEFRONT) and its shape; zero fields take the library defaults."""
+    _fields_ = [("kernel", ctypes.c_int), ("block_size", ctypes.c_int),
+                ("steps_per_refill", ctypes.c_uint32), ("queue_chunk", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class SortedTuning(ctypes.Structure):
     """optphot_sorted_tuning: zero fields take the library defaults."""
     _fields_ = [("slots_per_block", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
@@ -147,6 +161,11 @@ def _load():
                                                           P(_RecordsC), ctypes.c_uint32,
                                                           ctypes.c_int, ctypes.c_uint64,
                                                           P(EventStats), P(RunTiming)]),
+        "optphot_run_gpu_records_tuned": (ctypes.c_int, [P(SimParams), ctypes.c_uint64,
+                                                         P(_RecordsC), ctypes.c_int,
+                                                         P(RecordsTuning), ctypes.c_uint64,
+                                                         P(RecordsTuning), P(EventStats),
+                                                         P(RunTiming)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -320,6 +339,37 @@ def run_gpu_records_device(p: SimParams, n: int, steps_per_launch: int = 0,
                                              ctypes.byref(st), ctypes.byref(tm))
     _check(rc)
     return out, tm, st
+
+
+def run_gpu_records_tuned(p: SimParams, n: int, tuning: RecordsTuning | dict | None = None,
+                          first_id: int = 0, device: str | None = None,
+                          out: PhotonRecords | None = None):
+    """Records with a chosen launch shape (new; no reference counterpart):
+    `tuning` None (or kernel RECORDS_AUTO) lets the library choose the kernel
+    from n. Records go to host arrays (PhotonRecords), or, with `device` set,
+    into torch tensors on that device. Returns (records, RunTiming,
+    EventStats, the RecordsTuning that ran)."""
+    if isinstance(tuning, dict):
+        tuning = RecordsTuning(**tuning)
+    if device is not None:
+        import torch
+        dtypes = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint32): torch.uint32,
+                  np.dtype(np.float32): torch.float32}
+        rec = {f: torch.empty(n, dtype=dtypes[np.dtype(d)], device=device)
+               for f, d in zip(RECORD_FIELDS, RECORD_DTYPES)}
+        torch.cuda.synchronize(device)  # allocations complete before another stream writes
+        c = _RecordsC(*[rec[f].data_ptr() for f in RECORD_FIELDS])
+    else:
+        rec = out if out is not None else PhotonRecords.empty(n)
+        c = rec._c()
+    tm, st, chosen = RunTiming(), EventStats(), RecordsTuning()
+    rc = _lib.optphot_run_gpu_records_tuned(ctypes.byref(p), n, ctypes.byref(c),
+                                            1 if device is not None else 0,
+                                            ctypes.byref(tuning) if tuning is not None else None,
+                                            first_id, ctypes.byref(chosen), ctypes.byref(st),
+                                            ctypes.byref(tm))
+    _check(rc)
+    return rec, tm, st, chosen
 
 
 def run_gpu_tally_wavefront(p: SimParams, c: TallyConfig, n: int, first_id: int = 0,

@@ -544,6 +544,23 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
 
 enum class RecKernel { History, Wavefront };
 
+// Automatic records kernel: the wavefront kernel (kRecordsAutoSteps steps
+// between refills) once n reaches kRecordsAutoPhotonsPerLane photons per lane
+// of its persistent grid, else the history kernel.
+constexpr uint64_t kRecordsAutoPhotonsPerLane = 8;
+constexpr uint32_t kRecordsAutoSteps = 2;
+
+// Launch shape of a records run: the history kernel (block) or the wavefront
+// kernel (block, steps between refills, queue chunk, persistent blocks per CU;
+// 0 = the occupancy limit).
+struct RecordsShape {
+  RecKernel kind = RecKernel::History;
+  int block = 256;
+  uint32_t steps = 0;
+  uint32_t queue_chunk = 256;
+  int blocks_per_cu = 0;
+};
+
 constexpr size_t kRecordBytes = 29;                                  // backend.hpp:49
 constexpr uint64_t kStagePhotonsMax = (32ull << 20) / kRecordBytes;  // 32 MB per slot
 constexpr uint64_t kPieceMin = 1ull << 16;
@@ -638,9 +655,11 @@ void stage_records_to_host(Context& c, const DevRecords& buf, uint64_t n,
 }
 
 RecordsRun run_records(Context& c, const optphot_sim_params& p, uint64_t n,
-                       const optphot_records* host, const DevRecords* dev, RecKernel kind,
-                       int block, uint32_t steps, uint64_t first_id, uint64_t chunk,
+                       const optphot_records* host, const DevRecords* dev,
+                       const RecordsShape& shape, uint64_t first_id, uint64_t chunk,
                        unsigned long long* scratch, optphot_run_timing& tm) {
+  const RecKernel kind = shape.kind;
+  const int block = shape.block;
   RecordsRun run;
   if (n == 0) return run;
   const uint64_t limit = uint64_t{1} << 31;
@@ -677,7 +696,7 @@ RecordsRun run_records(Context& c, const optphot_sim_params& p, uint64_t n,
   LaunchOut lo{};
   lo.records = true;
   lo.counters = scratch ? scratch + kScratchCounters : nullptr;
-  const optphot_wavefront_tuning tu{block, steps, 256, 0};
+  const optphot_wavefront_tuning tu{block, shape.steps, shape.queue_chunk, shape.blocks_per_cu};
 
   const size_t sz[8] = {1, 4, 4, 4, 4, 4, 4, 4};
   struct Piece {
@@ -893,8 +912,9 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records(
     c.init();
     optphot_run_timing tm{};
     const auto t0 = std::chrono::steady_clock::now();
-    run_records(c, *p, n, out, nullptr, RecKernel::History, block_size, 0, first_id, chunk,
-                nullptr, tm);
+    RecordsShape rs;
+    rs.block = block_size;
+    run_records(c, *p, n, out, nullptr, rs, first_id, chunk, nullptr, tm);
     tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
   });
@@ -929,9 +949,11 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records_device(
       launch_zero_u64(scratch, kScratchSums, c.stream);
       HIPCHK(hipGetLastError());
     }
-    const RecordsRun r = run_records(
-        c, *p, n, nullptr, &d, steps_per_launch > 0 ? RecKernel::Wavefront : RecKernel::History,
-        block_size, steps_per_launch, first_id, 0, scratch, tm);
+    RecordsShape rs;
+    rs.kind = steps_per_launch > 0 ? RecKernel::Wavefront : RecKernel::History;
+    rs.block = block_size;
+    rs.steps = steps_per_launch;
+    const RecordsRun r = run_records(c, *p, n, nullptr, &d, rs, first_id, 0, scratch, tm);
     if (stats) {
       optphot_event_stats st{};
       st.launches = r.launches;
@@ -944,6 +966,82 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records_device(
       }
       *stats = st;
     }
+    tm.total_ms = ms_since(t0);
+    if (timing) *timing = tm;
+  });
+}
+
+__attribute__((visibility("default"))) int optphot_run_gpu_records_tuned(
+    const optphot_sim_params* p, uint64_t n, const optphot_records* out, int out_on_device,
+    const optphot_records_tuning* tuning, uint64_t first_id, optphot_records_tuning* chosen,
+    optphot_event_stats* stats, optphot_run_timing* timing) {
+  return guarded([&] {
+    validate_params(p);
+    optphot_records_tuning t{OPTPHOT_RECORDS_AUTO, 0, 0, 0, 0};
+    if (tuning) t = *tuning;
+    if (t.kernel < OPTPHOT_RECORDS_AUTO || t.kernel > OPTPHOT_RECORDS_WAVEFRONT)
+      throw InvalidArgument("records tuning: kernel must be OPTPHOT_RECORDS_AUTO, _HISTORY or "
+                            "_WAVEFRONT");
+    if (t.block_size == 0) t.block_size = 256;
+    if (t.steps_per_refill == 0) t.steps_per_refill = kRecordsAutoSteps;
+    if (t.queue_chunk == 0) t.queue_chunk = 256;
+    if (t.blocks_per_cu < 0) throw InvalidArgument("records tuning: blocks_per_cu must be >= 0");
+    validate_block(t.block_size, "records");
+    if (n > 0) validate_records(out);
+    Context& c = ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.init();
+    if (n > 0 && out_on_device) {
+      hipPointerAttribute_t attr{};
+      if (hipPointerGetAttributes(&attr, out->t) != hipSuccess ||
+          attr.type != hipMemoryTypeDevice)
+        throw InvalidArgument("records: out_on_device set but the arrays are not device memory");
+    }
+    optphot_run_timing tm{};
+    const auto t0 = std::chrono::steady_clock::now();
+    const optphot_wavefront_tuning wt{t.block_size, t.steps_per_refill, t.queue_chunk,
+                                      t.blocks_per_cu};
+    const WavefrontShape ws = wavefront_shape(c, &wt, true, false, 0, std::max<uint64_t>(n, 1));
+    if (t.kernel == OPTPHOT_RECORDS_AUTO) {
+      // Refilling finished lanes pays once each lane of the persistent grid
+      // has several photons to go through; below that every photon starts at
+      // once on the history kernel, and both end with the longest history.
+      const uint64_t lanes = static_cast<uint64_t>(ws.grid) * ws.block;
+      t.kernel = n >= kRecordsAutoPhotonsPerLane * lanes ? OPTPHOT_RECORDS_WAVEFRONT
+                                                        : OPTPHOT_RECORDS_HISTORY;
+    }
+    RecordsShape rs;
+    rs.kind = t.kernel == OPTPHOT_RECORDS_WAVEFRONT ? RecKernel::Wavefront : RecKernel::History;
+    rs.block = t.block_size;
+    rs.steps = t.steps_per_refill;
+    rs.queue_chunk = t.queue_chunk;
+    rs.blocks_per_cu = t.blocks_per_cu;
+    unsigned long long* scratch = nullptr;
+    if (rs.kind == RecKernel::Wavefront) {
+      scratch = ensure_scratch(c, ws.grid, tm.alloc_ms);
+      launch_zero_u64(scratch, kScratchSums, c.stream);  // queue + counters
+      HIPCHK(hipGetLastError());
+    }
+    DevRecords d{};
+    if (out_on_device)
+      d = DevRecords{out->fate, out->n_boundary, out->n_scatter, out->t,
+                     out->x,    out->y,          out->z,         out->path};
+    const RecordsRun r = run_records(c, *p, n, out_on_device ? nullptr : out,
+                                     out_on_device ? &d : nullptr, rs, first_id,
+                                     uint64_t{1} << 24, scratch, tm);
+    if (stats) {
+      optphot_event_stats st{};
+      st.launches = r.launches;
+      st.kernels_ms = r.kernel_ms;
+      if (scratch) {
+        optphot_step_counters cnt{};
+        read_counters(c, &cnt);
+        st.useful_lane_steps = cnt.v[kCntUseful];
+        st.issued_lane_steps = cnt.v[kCntIssued];
+      }
+      *stats = st;
+    }
+    if (chosen) *chosen = t;
     tm.total_ms = ms_since(t0);
     if (timing) *timing = tm;
   });
@@ -1093,8 +1191,11 @@ __attribute__((visibility("default"))) int optphot_run_gpu_records_event(
     unsigned long long* scratch = ensure_scratch(c, s.grid, tm.alloc_ms);
     launch_zero_u64(scratch, kScratchSums, c.stream);  // queue + counters
     HIPCHK(hipGetLastError());
-    const RecordsRun r = run_records(c, *p, n, out, nullptr, RecKernel::Wavefront, block_size,
-                                     steps_per_launch, first_id, chunk, scratch, tm);
+    RecordsShape rs;
+    rs.kind = RecKernel::Wavefront;
+    rs.block = block_size;
+    rs.steps = steps_per_launch;
+    const RecordsRun r = run_records(c, *p, n, out, nullptr, rs, first_id, chunk, scratch, tm);
     optphot_event_stats st{};
     st.launches = r.launches;
     st.kernels_ms = r.kernel_ms;

@@ -181,6 +181,38 @@ int optphot_run_gpu_records_device(const optphot_sim_params* p, uint64_t n,
                                    int block_size, uint64_t first_id, optphot_event_stats* stats,
                                    optphot_run_timing* timing);
 
+/* Launch shape of a records run (no counterpart in the reference). */
+enum {
+  OPTPHOT_RECORDS_AUTO = 0,      /* choose from n (see optphot_run_gpu_records_tuned) */
+  OPTPHOT_RECORDS_HISTORY = 1,   /* one lane per photon */
+  OPTPHOT_RECORDS_WAVEFRONT = 2  /* persistent grid, finished lanes refilled */
+};
+typedef struct optphot_records_tuning {
+  int kernel;                 /* OPTPHOT_RECORDS_* */
+  int block_size;             /* multiple of 64 in [64, 1024]; 0 = 256 */
+  uint32_t steps_per_refill;  /* wavefront: steps between refills; 0 = 2 */
+  uint32_t queue_chunk;       /* wavefront: photons a wave claims per atomic; 0 = 256 */
+  int blocks_per_cu;          /* wavefront: persistent blocks per CU; 0 = occupancy limit */
+} optphot_records_tuning;
+
+/* Records of photons first_id .. first_id+n-1 (the records of
+ * optphot_run_gpu_records, bit for bit) with a chosen launch shape, into host
+ * arrays (out_on_device = 0; pipelined through pinned staging as
+ * optphot_run_gpu_records) or caller-owned device arrays (out_on_device = 1).
+ * No counterpart in the reference, whose run_gpu_records / run_gpu_records_event
+ * (backend.hpp:88-89, 112-115) take the kernel's parameters from the caller:
+ * this entry point serves callers that take a tuned shape, or none.
+ * tuning NULL or kernel OPTPHOT_RECORDS_AUTO: the wavefront kernel (2 steps
+ * between refills, queue chunk 256, block size as given or 256) once n
+ * reaches 8 photons per lane of its persistent grid, else the history kernel.
+ * `chosen` (may be NULL) receives the shape that ran. stats / timing may be
+ * NULL. */
+int optphot_run_gpu_records_tuned(const optphot_sim_params* p, uint64_t n,
+                                  const optphot_records* out, int out_on_device,
+                                  const optphot_records_tuning* tuning, uint64_t first_id,
+                                  optphot_records_tuning* chosen, optphot_event_stats* stats,
+                                  optphot_run_timing* timing);
+
 /* Tally of photons first_id .. first_id+n-1 by the persistent wavefront kernel
  * (device work queue, refill of finished lanes, LDS-privatised histograms).
  * Same tally as optphot_run_gpu_tally, word for word. tuning / stats /

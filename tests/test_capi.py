@@ -83,3 +83,17 @@ def test_adjoint_rejects_reflectivity_outside_unit_interval():
     if not backend.gpu_available():
         with pytest.raises(RuntimeError, match="no HIP device"):
             adjoint.run_gpu_adjoint(b, np.ones((1, 9)), 8, 50.0)
+
+
+def test_records_tuned_rejects_bad_shapes_before_the_device():
+    """optphot_run_gpu_records_tuned checks the kernel choice and the shape
+    before it touches the device; a valid shape without a GPU reaches the
+    device check."""
+    for bad, what in ((dict(kernel=3), "kernel"), (dict(kernel=-1), "kernel"),
+                      (dict(kernel=2, block_size=100), "block"),
+                      (dict(kernel=2, blocks_per_cu=-1), "blocks_per_cu")):
+        with pytest.raises(ValueError, match=what):
+            backend.run_gpu_records_tuned(SimParams(), 10, bad)
+    if not backend.gpu_available():
+        with pytest.raises(RuntimeError, match="no HIP device"):
+            backend.run_gpu_records_tuned(SimParams(), 10, None)

@@ -215,3 +215,36 @@ def test_division_paths_at_range_limits(backend, oracle_portable, case):
     np.testing.assert_array_equal(t.fate_counts, rt["fate_counts"])
     np.testing.assert_array_equal(t.time_hist, rt["time_hist"])
     np.testing.assert_array_equal(t.xy_hist, rt["xy_hist"])
+
+
+def test_records_tuned_shapes_equal_history(backend, oracle_portable):
+    """optphot_run_gpu_records_tuned: every kernel and shape, into host or
+    device arrays, writes the oracle's records bit for bit; the automatic
+    choice takes the history kernel for a few photons per lane and the
+    wavefront kernel from 8 photons per lane of its grid."""
+    from gpu_optical_photons_amd.backend import RECORDS_HISTORY, RECORDS_WAVEFRONT
+    p = configs.PARITY["every_process"]
+    n = 300_000
+    ref = oracle_portable.records(p, n)
+    for tu in (None, dict(kernel=RECORDS_HISTORY, block_size=64),
+               dict(kernel=RECORDS_WAVEFRONT, block_size=128, steps_per_refill=1,
+                    queue_chunk=64, blocks_per_cu=1),
+               dict(kernel=RECORDS_WAVEFRONT, block_size=1024, steps_per_refill=8,
+                    queue_chunk=1024, blocks_per_cu=2)):
+        for device in (None, "cuda:0"):
+            rec, _, _, chosen = backend.run_gpu_records_tuned(p, n, tu, device=device)
+            if device is not None:
+                rec = {f: rec[f].cpu().numpy() for f in FIELDS}
+            assert_records_identical(rec, ref, f"{tu} {device}")
+            if tu is not None:
+                assert chosen.kernel == tu["kernel"]
+    _, _, _, small = backend.run_gpu_records_tuned(p, 1000, None, device="cuda:0")
+    assert small.kernel == RECORDS_HISTORY
+    big_n = 8_000_000
+    rec, _, st, big = backend.run_gpu_records_tuned(p, big_n, None, device="cuda:0")
+    assert big.kernel == RECORDS_WAVEFRONT and st.issued_lane_steps > 0
+    sample = np.arange(0, big_n, 997)
+    part = oracle_portable.records(p, big_n)
+    for f in FIELDS:
+        assert np.array_equal(rec[f].cpu().numpy()[sample].view(np.uint8),
+                              part[f][sample].view(np.uint8)), f
