@@ -242,3 +242,26 @@ def test_records_through_several_staging_pieces(backend, oracle_portable):
     _, r2, _, _ = run_gpu_gensteps(p, c, g[:300], first_id=5, records=True)
     for f in FIELDS:
         assert np.array_equal(getattr(r2, f).view(np.uint8), ref[f][:m].view(np.uint8)), f
+
+
+@pytest.mark.gpu
+def test_gpu_genstep_records_in_large_pieces(backend, oracle_portable):
+    """Genstep records above 8 x 2^18 photons: the staged copy pieces then
+    hold at least 2^18 photons each, which the host copies into the caller's
+    arrays on several threads, each field split into byte ranges (an odd
+    photon count makes the last range of the 1-byte fate field odd). Byte for
+    byte against the oracle."""
+    from gpu_optical_photons_amd.gensteps import run_gpu_gensteps
+    from gpu_optical_photons_amd.params import TallyConfig
+    p = PARITY["c2_long"]
+    g = make_gensteps(3000, p, photons_per_mm=200.0, seed=21, cherenkov_fraction=0.3)
+    g["n_photons"][0] += 1 - int(g["n_photons"].sum()) % 2  # odd total
+    total = int(g["n_photons"].sum())
+    assert total > 8 * (1 << 18) and total % 2 == 1
+    print(f"[gensteps] large pieces: {total} photons", flush=True)
+    c = TallyConfig(64, 100.0, 16)
+    _, ref_r = oracle_portable.gensteps(p, g, first_id=3, time_bins=64, time_max=100.0,
+                                        xy_bins=16, records=True)
+    _, r, _, _ = run_gpu_gensteps(p, c, g, first_id=3, records=True)
+    for f in FIELDS:
+        assert np.array_equal(getattr(r, f).view(np.uint8), ref_r[f].view(np.uint8)), f

@@ -62,6 +62,12 @@ def test_records_pipeline_many_pieces(backend, oracle_portable):
     ev, _, st = backend.run_gpu_records_event(p, n, 4, chunk=1_000_000)
     assert_records_identical(ev, ref, "wavefront pipeline")
     assert st.launches == 3
+    # one chunk of 3e6 + 1: copy pieces of 375 040 photons (>= 2^18), which the
+    # host splits over several threads; the odd count leaves odd byte ranges
+    print("[parity] records pipeline, large pieces", flush=True)
+    ref1 = oracle_portable.records(p, n + 1)
+    b, _ = backend.run_gpu_records(p, n + 1)
+    assert_records_identical(b, ref1, "large pieces")
 
 
 @pytest.mark.parametrize("k", [0, 1, 5])
