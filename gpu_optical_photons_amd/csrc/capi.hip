@@ -483,7 +483,7 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
 }
 
 WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, bool records,
-                               bool tally, size_t lds, uint64_t n) {
+                               bool tally, size_t lds, uint64_t n, int surface = -1) {
   optphot_wavefront_tuning t{256, 4, 256, 0, 0, 0};
   if (tu) {
     if (tu->block_size) t.block_size = tu->block_size;
@@ -498,7 +498,8 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   }
   if (t.successor_threshold > 64)
     throw InvalidArgument("wavefront: successor_threshold must be in [0, 64]");
-  if (t.stream_in_lds && ring_lds_bytes(static_cast<uint32_t>(lds / 4), t.block_size) > c.max_lds)
+  if (t.stream_in_lds &&
+      wavefront_ring_lds_bytes(static_cast<uint32_t>(lds / 4), t.block_size, surface) > c.max_lds)
     throw InvalidArgument("wavefront: tally plus LDS streams exceed the LDS of a block; use a "
                           "smaller block_size");
   if (t.sorted_phases &&
@@ -513,7 +514,7 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   const int max_per_cu =
       std::max(1, wavefront_blocks_per_cu(records, tally, t.block_size, lds,
                                           t.successor_threshold > 0, t.stream_in_lds != 0,
-                                          t.sorted_phases != 0));
+                                          t.sorted_phases != 0, surface));
   const int per_cu = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, max_per_cu) : max_per_cu;
   const uint64_t needed = (n + t.block_size - 1) / t.block_size;
   WavefrontShape s{};
@@ -1088,7 +1089,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_tally_wavefront(
     const DevTally T = to_dev_tally(*tc, *p);
     const size_t lds = static_cast<size_t>(T.words) * 4;
     check_lds(c, T);
-    const WavefrontShape s = wavefront_shape(c, tuning, false, true, lds, n);
+    const WavefrontShape s = wavefront_shape(c, tuning, false, true, lds, n,
+                                             static_cast<int>(p->surface));
     optphot_run_timing tm{};
     run_tally_common(c, *p, *tc, n, tally, first_id, true, s.block, s.grid, s, tm, stats,
                      counters);
@@ -1115,7 +1117,8 @@ __attribute__((visibility("default"))) int optphot_run_gpu_tally_wavefront_noall
     const DevTally T = to_dev_tally(*tc, *p);
     const size_t lds = static_cast<size_t>(T.words) * 4;
     check_lds(c, T);
-    WavefrontShape s = wavefront_shape(c, tuning, false, true, lds, n);
+    WavefrontShape s = wavefront_shape(c, tuning, false, true, lds, n,
+                                       static_cast<int>(p->surface));
     // Scratch layout (bytes): queue head and counters, 2 u64 sums per block,
     // then at the next 256-byte boundary the u32 partial rows of every block.
     constexpr size_t kCap = noalloc_store::kScratchBytes;

@@ -278,6 +278,14 @@ inline size_t ring_words(int block) {
 // LDS bytes of a wavefront tally launch with the streams in LDS: the ring
 // region and the tally words.
 size_t ring_lds_bytes(uint32_t tally_words, int block);
+// Fresnel cell-bound tables a wavefront ring launch keeps in LDS: the
+// readout's, and the outside medium's when the side faces are polished (or
+// the surface model is not fixed at compile time).
+constexpr uint32_t fresnel_tables(int surface) {
+  return (surface == static_cast<int>(kPolished) || surface < 0) ? 2u : 1u;
+}
+// LDS of a wavefront ring launch: ring region, tally, Fresnel cell bounds.
+size_t wavefront_ring_lds_bytes(uint32_t tally_words, int block, int surface);
 // Whether every kernel that places its ring region at LDS address 0 has no
 // static LDS, so that its dynamic LDS (and the region) starts at address 0.
 bool ring_kernels_start_at_lds_zero();
@@ -285,6 +293,7 @@ bool ring_kernels_start_at_lds_zero();
 // queue counters and 9 words per lane of queues and exchange slots.
 size_t phase_lds_bytes(uint32_t tally_words, int block);
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds,
-                            bool successor = false, bool ring = false, bool phases = false);
+                            bool successor = false, bool ring = false, bool phases = false,
+                            int surface = -1);
 
 }  // namespace ophip

@@ -10,8 +10,10 @@
 //  * the step is mostly branch-free: every lane computes the travelled
 //    distance d = min(absorb, scatter, wall) and advances with one update;
 //    Fresnel (readout or polished face) is evaluated once for all lanes that
-//    need it, and one rotate_about() serves both Rayleigh scattering and
-//    Lambertian reflection, so mixed waves do not execute those bodies twice;
+//    need it (and, in kernels that keep per-cell bounds of R, only for the
+//    few decisions the bounds leave open), and one rotate_about() serves both
+//    Rayleigh scattering and Lambertian reflection, so mixed waves do not
+//    execute those bodies twice;
 //  * specular reflection on an axis-aligned face is the exact negation of one
 //    direction component (d - 2 (d.n) n rounds to exactly that);
 //  * the Philox stream keeps an 8-word window (two blocks) that is topped up
@@ -730,6 +732,81 @@ __device__ __forceinline__ float fresnel_prob(const DevParams& P, bool readout, 
   return matched ? 0.0f : (tir ? 1.0f : R);
 }
 
+// Fresnel decisions from cell bounds. The decision of a readout or polished
+// hit is u <= prob with u = (k + 1) 2^-24 (k the top 24 bits of the drawn
+// word) and prob = fresnel_prob(cos_i). A kernel can keep, per interface, a
+// table of kFresnelCells words: word j bounds prob over the cos_i in
+// [j, j + 1] / kFresnelCells, packed as lo | hi << 16 with
+//   lo 2^-16 <= prob <= (hi + 1) 2^-16   for every such cos_i.
+// With k8 = word >> 16 (the top 16 bits of k): k8 < lo gives u <= lo 2^-16,
+// so the photon is reflected; k8 > hi gives u > (hi + 1) 2^-16, so it is
+// transmitted. Only when k8 lies in [lo, hi] is fresnel_prob evaluated, so
+// every decision is the one fresnel_prob gives.
+//
+// The bounds are exact for the float32 computation, not for the
+// mathematical R: every operation of fresnel_prob is a correctly rounded
+// multiply, add, subtract, square root, maximum or division by a positive
+// denominator, and each of these is monotone in each operand. Evaluating the
+// same operations on the end points of each operand's range (all four
+// corners of a quotient or difference) therefore bounds every value
+// fresnel_prob can produce in the cell, with its own rounding; operands that
+// depend on each other are only widened by this.
+constexpr uint32_t kFresnelCells = 512u;
+
+struct FInterval {
+  float lo, hi;
+};
+
+__device__ __forceinline__ FInterval iv_quot(FInterval a, FInterval b) {  // b.lo > 0
+  return FInterval{__builtin_fminf(a.lo / b.lo, a.lo / b.hi),
+                   __builtin_fmaxf(a.hi / b.lo, a.hi / b.hi)};
+}
+
+__device__ __forceinline__ FInterval iv_square(FInterval a) {
+  const float l = a.lo * a.lo, h = a.hi * a.hi;
+  if (a.lo >= 0.0f) return FInterval{l, h};
+  if (a.hi <= 0.0f) return FInterval{h, l};
+  return FInterval{0.0f, __builtin_fmaxf(l, h)};
+}
+
+// Packed bounds of fresnel_prob(cos_i) for cos_i in [j, j + 1] / kFresnelCells,
+// interface n1 -> n2 with eta2 = (n1 / n2)^2 as in DevParams. Uses IEEE
+// division and square root, which give the bits of div_n and sqrt_n wherever
+// fresnel_prob uses those.
+__device__ __forceinline__ uint32_t fresnel_cell_bounds(float n1, float n2, float eta2,
+                                                         bool matched, uint32_t j) {
+  constexpr uint32_t kNever = 0xffff0000u;  // lo = 0, hi = 0xffff: always evaluated
+  if (matched) return 0u;                    // prob = 0
+  const float cl = static_cast<float>(j) * (1.0f / kFresnelCells);  // exact
+  const float ch = static_cast<float>(j + 1u) * (1.0f / kFresnelCells);
+  // sin2_t = eta2 max(0, 1 - c^2) does not increase with c
+  const float s_hi = eta2 * __builtin_fmaxf(0.0f, 1.0f - cl * cl);
+  const float s_lo = eta2 * __builtin_fmaxf(0.0f, 1.0f - ch * ch);
+  if (s_lo >= 1.0f) return 0xffffffffu;  // total internal reflection throughout: prob = 1
+  // cos_i of the cell outside total internal reflection: sin2_t <= 1 - 2^-24
+  const float sn_hi = __builtin_fminf(s_hi, 0x1.fffffep-1f);
+  const FInterval ct{__builtin_sqrtf(1.0f - sn_hi), __builtin_sqrtf(1.0f - s_lo)};
+  const FInterval n1c{n1 * cl, n1 * ch}, n2c{n2 * cl, n2 * ch};
+  const FInterval n1t{n1 * ct.lo, n1 * ct.hi}, n2t{n2 * ct.lo, n2 * ct.hi};
+  const FInterval rs = iv_quot(FInterval{n1c.lo - n2t.hi, n1c.hi - n2t.lo},
+                               FInterval{n1c.lo + n2t.lo, n1c.hi + n2t.hi});
+  const FInterval rp = iv_quot(FInterval{n2c.lo - n1t.hi, n2c.hi - n1t.lo},
+                               FInterval{n2c.lo + n1t.lo, n2c.hi + n1t.hi});
+  const FInterval s2 = iv_square(rs), p2 = iv_square(rp);
+  float lo = 0.5f * (s2.lo + p2.lo), hi = 0.5f * (s2.hi + p2.hi);
+  if (s_hi >= 1.0f) {  // part of the cell is totally reflected (prob = 1)
+    lo = __builtin_fminf(lo, 1.0f);
+    hi = __builtin_fmaxf(hi, 1.0f);
+  }
+  if (!(lo == lo) || !(hi == hi)) return kNever;
+  // lo16 = floor(lo 2^16), hi16 = ceil(hi 2^16) - 1, both clamped to [0, 0xffff]
+  const float fl = __builtin_floorf(__builtin_fminf(__builtin_fmaxf(lo, 0.0f), 1.0f) * 65536.0f);
+  const float fh = __builtin_ceilf(__builtin_fminf(__builtin_fmaxf(hi, 0.0f), 1.0f) * 65536.0f);
+  const uint32_t l16 = fl >= 65535.0f ? 0xffffu : static_cast<uint32_t>(fl);
+  const uint32_t h16 = fh < 1.0f ? 0u : (fh >= 65536.0f ? 0xffffu : static_cast<uint32_t>(fh) - 1u);
+  return l16 | (h16 << 16);
+}
+
 // Specular reflection on an axis-aligned face: d - 2 (d.n) n rounds to
 // exactly the negation of the approached component.
 __device__ __forceinline__ void flip_axis(V3& d, int axis) {
@@ -760,10 +837,14 @@ __device__ __forceinline__ V3 redirect(const DevParams& P, bool scatter, V3 d, i
 // is alive; otherwise *fate is set. The stream must hold the step's (up to 5)
 // words: Stream after refill() (pos <= 3), RingStream with level() >= 5.
 // kSurf >= 0 fixes the surface model at compile time (a launch whose events
-// all share it); kSurf < 0 reads it from P.
+// all share it); kSurf < 0 reads it from P. `fres` (forward steps only):
+// Fresnel cell bounds, kFresnelCells words for the readout followed, when the
+// side faces are polished, by kFresnelCells words for the outside medium; a
+// decision they settle skips fresnel_prob.
 template <int kSurf = -1, bool kAdj = false, class RNG>
 __device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, uint32_t& fate,
-                                     StepInfo& info, AdjState* adj = nullptr) {
+                                     StepInfo& info, AdjState* adj = nullptr,
+                                     const uint32_t* fres = nullptr) {
   const uint32_t surface = kSurf >= 0 ? static_cast<uint32_t>(kSurf) : P.surface;
   const Advance a = advance(P, rng, ph);
   const bool absorb = a.absorb, scatter = a.scatter;
@@ -777,10 +858,22 @@ __device__ __forceinline__ bool step(const DevParams& P, RNG& rng, Photon& ph, u
   // --- Fresnel, once for readout and polished lanes --------------------------
   float prob = P.reflectivity;
   bool scored = false;  // a Fresnel decision with a score (adjoint only)
-  if (readout || polished) prob = fresnel_prob(P, readout, a.cos_i, scored);
+  bool sure_r = false, sure_t = false;  // settled by the cell bounds
+  if (!kAdj && fres != nullptr && (readout || polished)) {
+    // cos_i in (0, 1]: cos_i * kFresnelCells is exact, its floor the cell
+    const uint32_t c = static_cast<uint32_t>(a.cos_i * static_cast<float>(kFresnelCells));
+    const uint32_t cell = (c < kFresnelCells ? c : kFresnelCells - 1u) + (readout ? 0u : kFresnelCells);
+    const uint32_t b = fres[cell];
+    const uint32_t k8 = rng.template peek<2>() >> 16;
+    sure_r = k8 < (b & 0xffffu);
+    sure_t = k8 > (b >> 16);
+  }
+  if ((readout || polished) && !(sure_r || sure_t))
+    prob = fresnel_prob(P, readout, a.cos_i, scored);
   // --- the surface decision draw (u <= prob survives, transport.hpp:41) -----
   const bool decide = readout || polished || refl;
-  const bool survived = decide && to_unit(rng.template peek<2>()) <= prob;
+  const bool survived =
+      decide && (sure_r || (!sure_t && to_unit(rng.template peek<2>()) <= prob));
   const bool lambert_reflect = refl && lambert && survived;
   if (kAdj && scored && (survived || !readout)) {
     // Bernoulli score: d log R (reflected) or d log (1 - R) (transmitted);

@@ -522,6 +522,18 @@ __global__ __launch_bounds__(1024) void wavefront_next_kernel(DevParams P0, Phil
   flush_block<true, kCount>(out, lds, snb, sns, wc);
 }
 
+// Fresnel cell bounds of the run (fresnel_cell_bounds) into LDS, filled by
+// the block: the readout's table, then the outside medium's when `tables` is 2.
+__device__ __forceinline__ void fill_fresnel_bounds(const DevParams& P, uint32_t* fres,
+                                                    uint32_t tables) {
+  for (uint32_t j = threadIdx.x; j < tables * kFresnelCells; j += blockDim.x) {
+    const bool ro = j < kFresnelCells;
+    fres[j] = fresnel_cell_bounds(P.n_scint, ro ? P.n_det : P.n_out, ro ? P.eta2_det : P.eta2_out,
+                                  ro ? P.det_matched != 0u : P.out_matched != 0u,
+                                  j & (kFresnelCells - 1u));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Persistent wavefront kernel with the photons' random streams in LDS (tally).
 // Each lane's Philox window is a 16-word ring in LDS (RingStream). Before a
@@ -543,9 +555,12 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
   // LDS: the ring region first (at address 0, a multiple of its size), then
-  // the block's tally
+  // the block's tally, then the Fresnel cell bounds (readout, and the outside
+  // medium for polished faces)
   uint32_t* const tal = lds + ring_region_words();
+  uint32_t* const fres = tal + ((out.tally.words + 3u) & ~3u);
   zero_lds(tal, out.tally.words);
+  fill_fresnel_bounds(P, fres, fresnel_tables(kSurf));
   __syncthreads();
   const uint32_t lane = lane_id();
   WaveCounters<kCount> wc;
@@ -630,7 +645,7 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
     StepInfo info{};
     bool finished = false;
     if (alive) {
-      const bool still = step<kSurf>(P, rng, ph, fate, info);
+      const bool still = step<kSurf>(P, rng, ph, fate, info, nullptr, fres);
       ++done;
       if (!still || done >= P.max_steps) {
         if (still) fate = kMaxSteps;
@@ -1766,8 +1781,8 @@ void launch_wavefront_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_i
                              const WavefrontShape& s, unsigned long long* queue,
                              const LaunchOut& out, hipStream_t stream) {
   hipLaunchKernelGGL((wavefront_ring_kernel<C, S>), dim3(s.grid), dim3(s.block),
-                     ring_lds_bytes(out.tally.words, s.block), stream, P, key, first_id, n,
-                     s.steps_per_refill, s.queue_chunk, queue, out);
+                     wavefront_ring_lds_bytes(out.tally.words, s.block, S), stream, P, key,
+                     first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
 }
 
 template <bool C, int S>
@@ -1884,6 +1899,11 @@ int blocks_per_cu(K kernel, int block, size_t lds) {
 size_t ring_lds_bytes(uint32_t tally_words, int block) {
   return (((static_cast<size_t>(tally_words) + 3u) & ~size_t{3}) + ring_words(block)) *
          sizeof(uint32_t);
+}
+
+size_t wavefront_ring_lds_bytes(uint32_t tally_words, int block, int surface) {
+  return ring_lds_bytes(tally_words, block) +
+         static_cast<size_t>(fresnel_tables(surface)) * kFresnelCells * sizeof(uint32_t);
 }
 
 bool ring_kernels_start_at_lds_zero() {
@@ -2085,14 +2105,15 @@ int history_blocks_per_cu(bool tally, int block, size_t lds) {
 }
 
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds, bool successor,
-                            bool ring, bool phases) {
+                            bool ring, bool phases, int surface) {
   if (records) return blocks_per_cu(wavefront_kernel<true, false, true>, block, lds);
   if (phases)
     return blocks_per_cu(wavefront_phase_kernel<false, kLambertian>, block,
                          phase_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
   if (ring)
     return blocks_per_cu(wavefront_ring_kernel<false, kLambertian>, block,
-                         ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
+                         wavefront_ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)),
+                                                  block, surface));
   if (successor) return blocks_per_cu(wavefront_next_kernel<false, kLambertian>, block, lds);
   return blocks_per_cu(wavefront_kernel<false, true, false, kLambertian>, block, lds);
 }
