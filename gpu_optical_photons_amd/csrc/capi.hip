@@ -415,8 +415,17 @@ void read_counters(Context& c, optphot_step_counters* counters) {
 
 // ---- the runs ---------------------------------------------------------------
 
-int history_grid_for_tally(Context& c, int block, size_t lds, uint64_t n) {
-  const int per_cu = std::max(1, history_blocks_per_cu(true, block, lds));
+// Whether a history launch (tally of `tally_words`, 0 for records) keeps the
+// Fresnel cell bounds: whenever they fit beside the tally.
+bool history_bounds(const Context& c, uint32_t tally_words, int surface) {
+  return history_lds_bytes(tally_words, true, surface) <= c.max_lds;
+}
+
+int history_grid_for_tally(Context& c, int block, uint32_t tally_words, int surface, uint64_t n) {
+  const bool bounds = history_bounds(c, tally_words, surface);
+  const int per_cu =
+      std::max(1, history_blocks_per_cu(true, block,
+                                        history_lds_bytes(tally_words, bounds, surface)));
   const uint64_t needed = (n + block - 1) / block;
   return static_cast<int>(std::max<uint64_t>(
       1, std::min<uint64_t>(needed, static_cast<uint64_t>(per_cu) * c.cu_count)));
@@ -446,6 +455,8 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
   out.counters = (counters || stats) ? scratch + kScratchCounters : nullptr;
   out.tally_on = true;
   out.records = false;
+  out.fresnel_bounds = wavefront ? shape.fresnel_bounds
+                                 : history_bounds(c, T.words, static_cast<int>(p.surface));
 
   launch_zero_u64(scratch, kScratchSums + 2 * static_cast<size_t>(grid), c.stream);
   launch_zero_u64(c.tally.as<unsigned long long>(), static_cast<size_t>(T.words) + 2, c.stream);
@@ -499,7 +510,8 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   if (t.successor_threshold > 64)
     throw InvalidArgument("wavefront: successor_threshold must be in [0, 64]");
   if (t.stream_in_lds &&
-      wavefront_ring_lds_bytes(static_cast<uint32_t>(lds / 4), t.block_size, surface) > c.max_lds)
+      wavefront_ring_lds_bytes(static_cast<uint32_t>(lds / 4), t.block_size, surface, false) >
+          c.max_lds)
     throw InvalidArgument("wavefront: tally plus LDS streams exceed the LDS of a block; use a "
                           "smaller block_size");
   if (t.sorted_phases &&
@@ -511,10 +523,18 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
     throw InvalidArgument("wavefront: queue_chunk must be a multiple of 64 in [64, 2^24]");
   if (t.steps_per_refill == 0)
     throw InvalidArgument("wavefront: steps_per_refill must be >= 1");
+  // Fresnel cell bounds: kept by the ring and register-stream kernels when
+  // they fit beside the tally (and ring)
+  const uint32_t words = static_cast<uint32_t>(lds / 4);
+  const bool bounds =
+      t.sorted_phases == 0 && (t.stream_in_lds != 0 || t.successor_threshold == 0) &&
+      (t.stream_in_lds != 0
+           ? wavefront_ring_lds_bytes(words, t.block_size, surface, true) <= c.max_lds
+           : history_lds_bytes(words, true, surface) <= c.max_lds);
   const int max_per_cu =
       std::max(1, wavefront_blocks_per_cu(records, tally, t.block_size, lds,
                                           t.successor_threshold > 0, t.stream_in_lds != 0,
-                                          t.sorted_phases != 0, surface));
+                                          t.sorted_phases != 0, surface, bounds));
   const int per_cu = t.blocks_per_cu > 0 ? std::min(t.blocks_per_cu, max_per_cu) : max_per_cu;
   const uint64_t needed = (n + t.block_size - 1) / t.block_size;
   WavefrontShape s{};
@@ -526,6 +546,7 @@ WavefrontShape wavefront_shape(Context& c, const optphot_wavefront_tuning* tu, b
   s.successor_threshold = t.successor_threshold;
   s.stream_in_lds = t.stream_in_lds;
   s.sorted_phases = t.sorted_phases;
+  s.fresnel_bounds = bounds;
   return s;
 }
 
@@ -715,9 +736,11 @@ RecordsRun run_records(Context& c, const optphot_sim_params& p, uint64_t n,
     HIPCHK(hipEventRecord(kev[2 * k], c.stream));
     if (kind == RecKernel::History) {
       const int grid = static_cast<int>((m + block - 1) / block);
+      lo.fresnel_bounds = history_bounds(c, 0, static_cast<int>(P.surface));
       launch_history(P, key, first_id + off, m, block, grid, lo, c.stream);
     } else {
       const WavefrontShape sm = wavefront_shape(c, &tu, true, false, 0, m);
+      lo.fresnel_bounds = sm.fresnel_bounds;
       launch_zero_u64(scratch + kScratchQueue, 1, c.stream);
       launch_wavefront(P, key, first_id + off, m, sm, scratch + kScratchQueue, lo, c.stream);
     }
@@ -1066,9 +1089,9 @@ __attribute__((visibility("default"))) int optphot_run_gpu_tally_counters(
     std::lock_guard<std::mutex> lock(c.mu);
     c.init();
     const DevTally T = to_dev_tally(*tc, *p);
-    const size_t lds = static_cast<size_t>(T.words) * 4;
     check_lds(c, T);
-    const int grid = history_grid_for_tally(c, block_size, lds, n);
+    const int grid =
+        history_grid_for_tally(c, block_size, T.words, static_cast<int>(p->surface), n);
     optphot_run_timing tm{};
     run_tally_common(c, *p, *tc, n, tally, first_id, false, block_size, grid, WavefrontShape{},
                      tm, nullptr, counters);
@@ -1156,6 +1179,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_tally_wavefront_noall
     out.counters = nullptr;
     out.tally_on = true;
     out.records = false;
+    out.fresnel_bounds = s.fresnel_bounds;
     // The array keeps the previous call's contents: clear the queue head and
     // the block sums (each block writes its whole partial row).
     launch_zero_u64(words64, kScratchSums + 2 * static_cast<size_t>(s.grid), c.stream);

@@ -72,6 +72,9 @@ struct LaunchOut {
   unsigned long long* counters;  // [kCntSlots], accumulated
   DevEventTally evt;           // used when `events`
   bool records, tally_on, events;
+  // History, register-stream and ring wavefront launches of one parameter
+  // set: keep the Fresnel cell bounds in LDS (set when they fit).
+  bool fresnel_bounds;
 };
 
 // Genstep launch: photons come from gensteps (device array), handed out as
@@ -115,6 +118,8 @@ struct WavefrontShape {
   // Tally launches: ring streams plus each step's Fresnel and rotation work
   // sorted onto block-wide LDS queues (takes precedence over both above).
   uint32_t sorted_phases = 0;
+  // The launch keeps the Fresnel cell bounds in LDS (LaunchOut::fresnel_bounds).
+  bool fresnel_bounds = false;
 };
 
 // Slots per claim for a slot queue of n_slots drained by `waves` waves: up to
@@ -267,7 +272,7 @@ void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
                        hipStream_t stream);
 
 // Occupancy queries (blocks per CU for a block size and dynamic LDS bytes).
-int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);
+int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);  // all dynamic LDS
 // Words of the LDS ring region of a block: 16 slots of the block size rounded
 // up to a power of two (the device side's ring_region_words()).
 inline size_t ring_words(int block) {
@@ -278,22 +283,32 @@ inline size_t ring_words(int block) {
 // LDS bytes of a wavefront tally launch with the streams in LDS: the ring
 // region and the tally words.
 size_t ring_lds_bytes(uint32_t tally_words, int block);
-// Fresnel cell-bound tables a wavefront ring launch keeps in LDS: the
-// readout's, and the outside medium's when the side faces are polished (or
-// the surface model is not fixed at compile time).
+// Fresnel cell-bound tables a launch keeps in LDS: the readout's, and the
+// outside medium's when the side faces are polished (or the surface model is
+// not fixed at compile time).
 constexpr uint32_t fresnel_tables(int surface) {
   return (surface == static_cast<int>(kPolished) || surface < 0) ? 2u : 1u;
 }
-// LDS of a wavefront ring launch: ring region, tally, Fresnel cell bounds.
-size_t wavefront_ring_lds_bytes(uint32_t tally_words, int block, int surface);
+// Bytes of those tables (0 when the launch does not keep them).
+constexpr size_t fresnel_lds_bytes(bool on, int surface) {
+  return on ? static_cast<size_t>(fresnel_tables(surface)) * kFresnelCells * 4u : 0u;
+}
+// LDS of a history or register-stream wavefront launch: the tally (0 words
+// for records), then the Fresnel cell bounds when `bounds`.
+size_t history_lds_bytes(uint32_t tally_words, bool bounds, int surface);
+// LDS of a wavefront ring launch: ring region, tally, Fresnel cell bounds
+// when `bounds`.
+size_t wavefront_ring_lds_bytes(uint32_t tally_words, int block, int surface, bool bounds);
 // Whether every kernel that places its ring region at LDS address 0 has no
 // static LDS, so that its dynamic LDS (and the region) starts at address 0.
 bool ring_kernels_start_at_lds_zero();
 // LDS bytes of the sorted-phase kernel: the ring layout, then the per-parity
 // queue counters and 9 words per lane of queues and exchange slots.
 size_t phase_lds_bytes(uint32_t tally_words, int block);
+// `lds`: the tally bytes; the function adds the ring region, phase queues and
+// Fresnel bounds (`bounds`, for `surface`) of the kernel selected.
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds,
                             bool successor = false, bool ring = false, bool phases = false,
-                            int surface = -1);
+                            int surface = -1, bool bounds = false);
 
 }  // namespace ophip

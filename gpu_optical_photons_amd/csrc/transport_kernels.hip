@@ -240,6 +240,18 @@ __device__ __forceinline__ void flush_block(const LaunchOut& out, uint32_t* lds,
   }
 }
 
+// Fresnel cell bounds of the run (fresnel_cell_bounds) into LDS, filled by
+// the block: the readout's table, then the outside medium's when `tables` is 2.
+__device__ __forceinline__ void fill_fresnel_bounds(const DevParams& P, uint32_t* fres,
+                                                    uint32_t tables) {
+  for (uint32_t j = threadIdx.x; j < tables * kFresnelCells; j += blockDim.x) {
+    const bool ro = j < kFresnelCells;
+    fres[j] = fresnel_cell_bounds(P.n_scint, ro ? P.n_det : P.n_out, ro ? P.eta2_det : P.eta2_out,
+                                  ro ? P.det_matched != 0u : P.out_matched != 0u,
+                                  j & (kFresnelCells - 1u));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // History-based kernel. Records: the grid covers n, lane i <-> photon i.
 // Tally: waves stride over the photons in steps of the whole grid.
@@ -248,10 +260,14 @@ __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey ke
                                                        uint64_t first_id, uint64_t n,
                                                        LaunchOut out) {
   extern __shared__ uint32_t lds[];
-  if (kTally) {
-    zero_lds(lds, out.tally.words);
-    __syncthreads();
-  }
+  // LDS: the tally (if any), then the Fresnel cell bounds (when the launch
+  // keeps them; never for the short per-event launches of a batch)
+  uint32_t* const fres = (!kEvt && out.fresnel_bounds)
+                             ? lds + (kTally ? (out.tally.words + 3u) & ~3u : 0u)
+                             : nullptr;
+  if (kTally) zero_lds(lds, out.tally.words);
+  if (fres) fill_fresnel_bounds(P, fres, fresnel_tables(static_cast<int>(P.surface)));
+  if (kTally || fres) __syncthreads();
   const uint32_t lane = lane_id();
   const uint64_t wave0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + (threadIdx.x - lane);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -279,7 +295,7 @@ __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey ke
       StepInfo info{};
       if (alive) {
         rng.refill(key);
-        const bool still = step(P, rng, ph, fate, info);
+        const bool still = step(P, rng, ph, fate, info, nullptr, fres);
         ++done;
         alive = still && done < P.max_steps;
         if (still && !alive) fate = kMaxSteps;
@@ -303,10 +319,13 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
                                                          LaunchOut out) {
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
-  if (kTally) {
-    zero_lds(lds, out.tally.words);
-    __syncthreads();
-  }
+  // LDS: the tally (if any), then the Fresnel cell bounds (if kept)
+  uint32_t* const fres =
+      out.fresnel_bounds ? lds + (kTally ? (out.tally.words + 3u) & ~3u : 0u) : nullptr;
+  if (kTally) zero_lds(lds, out.tally.words);
+  if (fres)
+    fill_fresnel_bounds(P, fres, fresnel_tables(kSurf >= 0 ? kSurf : static_cast<int>(P.surface)));
+  if (kTally || fres) __syncthreads();
   const uint32_t lane = lane_id();
   WaveCounters<kCount> wc;
   wc.clear();
@@ -374,7 +393,7 @@ __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey
       bool finished = false;
       if (alive) {
         rng.refill(key);
-        const bool still = step<kSurf>(P, rng, ph, fate, info);
+        const bool still = step<kSurf>(P, rng, ph, fate, info, nullptr, fres);
         ++done;
         if (!still || done >= P.max_steps) {
           if (still) fate = kMaxSteps;
@@ -522,18 +541,6 @@ __global__ __launch_bounds__(1024) void wavefront_next_kernel(DevParams P0, Phil
   flush_block<true, kCount>(out, lds, snb, sns, wc);
 }
 
-// Fresnel cell bounds of the run (fresnel_cell_bounds) into LDS, filled by
-// the block: the readout's table, then the outside medium's when `tables` is 2.
-__device__ __forceinline__ void fill_fresnel_bounds(const DevParams& P, uint32_t* fres,
-                                                    uint32_t tables) {
-  for (uint32_t j = threadIdx.x; j < tables * kFresnelCells; j += blockDim.x) {
-    const bool ro = j < kFresnelCells;
-    fres[j] = fresnel_cell_bounds(P.n_scint, ro ? P.n_det : P.n_out, ro ? P.eta2_det : P.eta2_out,
-                                  ro ? P.det_matched != 0u : P.out_matched != 0u,
-                                  j & (kFresnelCells - 1u));
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Persistent wavefront kernel with the photons' random streams in LDS (tally).
 // Each lane's Philox window is a 16-word ring in LDS (RingStream). Before a
@@ -555,12 +562,12 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
   // LDS: the ring region first (at address 0, a multiple of its size), then
-  // the block's tally, then the Fresnel cell bounds (readout, and the outside
-  // medium for polished faces)
+  // the block's tally, then the Fresnel cell bounds if kept (readout, and the
+  // outside medium for polished faces)
   uint32_t* const tal = lds + ring_region_words();
-  uint32_t* const fres = tal + ((out.tally.words + 3u) & ~3u);
+  uint32_t* const fres = out.fresnel_bounds ? tal + ((out.tally.words + 3u) & ~3u) : nullptr;
   zero_lds(tal, out.tally.words);
-  fill_fresnel_bounds(P, fres, fresnel_tables(kSurf));
+  if (fres) fill_fresnel_bounds(P, fres, fresnel_tables(kSurf));
   __syncthreads();
   const uint32_t lane = lane_id();
   WaveCounters<kCount> wc;
@@ -1771,7 +1778,8 @@ __global__ __launch_bounds__(256) void valu_probe_kernel(float* out, uint32_t it
 template <bool R, bool T, bool E, bool C>
 void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n, int block,
                       int grid, const LaunchOut& out, hipStream_t stream) {
-  const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
+  const size_t lds = history_lds_bytes(T ? out.tally.words : 0u, !E && out.fresnel_bounds,
+                                       static_cast<int>(P.surface));
   hipLaunchKernelGGL((history_kernel<R, T, E, C>), dim3(grid), dim3(block), lds, stream, P, key,
                      first_id, n, out);
 }
@@ -1781,7 +1789,8 @@ void launch_wavefront_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_i
                              const WavefrontShape& s, unsigned long long* queue,
                              const LaunchOut& out, hipStream_t stream) {
   hipLaunchKernelGGL((wavefront_ring_kernel<C, S>), dim3(s.grid), dim3(s.block),
-                     wavefront_ring_lds_bytes(out.tally.words, s.block, S), stream, P, key,
+                     wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds),
+                     stream, P, key,
                      first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
 }
 
@@ -1807,7 +1816,8 @@ template <bool R, bool T, bool C, int S = -1>
 void launch_wavefront_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                         const WavefrontShape& s, unsigned long long* queue, const LaunchOut& out,
                         hipStream_t stream) {
-  const size_t lds = T ? static_cast<size_t>(out.tally.words) * sizeof(uint32_t) : 0;
+  const size_t lds = history_lds_bytes(T ? out.tally.words : 0u, out.fresnel_bounds,
+                                       S >= 0 ? S : static_cast<int>(P.surface));
   hipLaunchKernelGGL((wavefront_kernel<R, T, C, S>), dim3(s.grid), dim3(s.block), lds, stream,
                      P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
 }
@@ -1901,9 +1911,13 @@ size_t ring_lds_bytes(uint32_t tally_words, int block) {
          sizeof(uint32_t);
 }
 
-size_t wavefront_ring_lds_bytes(uint32_t tally_words, int block, int surface) {
-  return ring_lds_bytes(tally_words, block) +
-         static_cast<size_t>(fresnel_tables(surface)) * kFresnelCells * sizeof(uint32_t);
+size_t history_lds_bytes(uint32_t tally_words, bool bounds, int surface) {
+  return ((static_cast<size_t>(tally_words) + 3u) & ~size_t{3}) * sizeof(uint32_t) +
+         fresnel_lds_bytes(bounds, surface);
+}
+
+size_t wavefront_ring_lds_bytes(uint32_t tally_words, int block, int surface, bool bounds) {
+  return ring_lds_bytes(tally_words, block) + fresnel_lds_bytes(bounds, surface);
 }
 
 bool ring_kernels_start_at_lds_zero() {
@@ -2105,17 +2119,19 @@ int history_blocks_per_cu(bool tally, int block, size_t lds) {
 }
 
 int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds, bool successor,
-                            bool ring, bool phases, int surface) {
-  if (records) return blocks_per_cu(wavefront_kernel<true, false, true>, block, lds);
+                            bool ring, bool phases, int surface, bool bounds) {
+  const size_t all =
+      history_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), bounds, surface);
+  if (records) return blocks_per_cu(wavefront_kernel<true, false, true>, block, all);
   if (phases)
     return blocks_per_cu(wavefront_phase_kernel<false, kLambertian>, block,
                          phase_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
   if (ring)
     return blocks_per_cu(wavefront_ring_kernel<false, kLambertian>, block,
                          wavefront_ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)),
-                                                  block, surface));
+                                                  block, surface, bounds));
   if (successor) return blocks_per_cu(wavefront_next_kernel<false, kLambertian>, block, lds);
-  return blocks_per_cu(wavefront_kernel<false, true, false, kLambertian>, block, lds);
+  return blocks_per_cu(wavefront_kernel<false, true, false, kLambertian>, block, all);
 }
 
 template <int S, bool U>
