@@ -1623,8 +1623,13 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
     if (ring && ring_lds_bytes(T.words, tu.block_size) > c.max_lds)
       throw InvalidArgument("gensteps: tally plus LDS streams exceed the LDS of a block; use a "
                             "smaller block_size");
-    const int max_per_cu =
-        std::max(1, genstep_blocks_per_cu(records != nullptr, tu.block_size, lds, ring));
+    // Fresnel cell bounds in LDS whenever they fit
+    const int surf = static_cast<int>(p->surface);
+    const bool bounds =
+        (ring ? wavefront_ring_lds_bytes(T.words, tu.block_size, surf, true)
+              : history_lds_bytes(T.words, true, surf)) <= c.max_lds;
+    const int max_per_cu = std::max(
+        1, genstep_blocks_per_cu(records != nullptr, tu.block_size, lds, ring, surf, bounds));
     const int per_cu = tu.blocks_per_cu > 0 ? std::min(tu.blocks_per_cu, max_per_cu) : max_per_cu;
     WavefrontShape s{};
     s.block = tu.block_size;
@@ -1634,6 +1639,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
     s.steps_per_refill = tu.steps_per_refill;
     s.queue_chunk = tu.queue_chunk;
     s.stream_in_lds = ring ? 1u : 0u;
+    s.fresnel_bounds = bounds;
     s.slots_per_claim = slots_per_claim(slots.size(), static_cast<uint64_t>(s.grid) * s.block / 64u);
     if (total / static_cast<uint64_t>(s.grid) >= 0xffffffffull)
       throw std::runtime_error("too many photons for one tally launch; split the run");
@@ -1660,6 +1666,7 @@ __attribute__((visibility("default"))) int optphot_run_gpu_gensteps(
     LaunchOut lo{};
     lo.tally = T;
     lo.tally_on = true;
+    lo.fresnel_bounds = s.fresnel_bounds;
     lo.records = records != nullptr && total > 0;
     lo.rec = rec;
     lo.partials = c.partials.as<uint32_t>();

@@ -255,7 +255,8 @@ void launch_rayleigh_table(float* table, hipStream_t stream);
 void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,
                      const WavefrontShape& shape, const GenstepArgs& g,
                      unsigned long long* queue, const LaunchOut& out, hipStream_t stream);
-int genstep_blocks_per_cu(bool records, int block, size_t lds, bool ring = false);
+int genstep_blocks_per_cu(bool records, int block, size_t lds, bool ring = false,
+                          int surface = -1, bool bounds = false);
 
 // Per-set chi-square loss of the batch tally's arrival-time histograms
 // (rows of `words` u32: 5 fates, then bins1 time bins) against target

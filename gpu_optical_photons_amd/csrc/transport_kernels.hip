@@ -1104,7 +1104,10 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
                                                        LaunchOut out) {
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
+  // LDS: the tally, then the Fresnel cell bounds (if kept)
+  uint32_t* const fres = out.fresnel_bounds ? lds + ((out.tally.words + 3u) & ~3u) : nullptr;
   zero_lds(lds, out.tally.words);
+  if (fres) fill_fresnel_bounds(P, fres, fresnel_tables(kSurf));
   __syncthreads();
   const uint32_t lane = lane_id();
   WaveCounters<kCount> wc;
@@ -1181,7 +1184,7 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
       bool finished = false;
       if (alive) {
         rng.refill(key);
-        const bool still = step<kSurf>(P, rng, ph, fate, info);
+        const bool still = step<kSurf>(P, rng, ph, fate, info, nullptr, fres);
         ++done;
         if (!still || done >= P.max_steps) {
           if (still) fate = kMaxSteps;
@@ -1214,9 +1217,11 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
   const DevParams& P = P0;
   extern __shared__ uint32_t lds[];
   // LDS: the ring region first (at address 0, a multiple of its size), then
-  // the block's tally
+  // the block's tally, then the Fresnel cell bounds if kept
   uint32_t* const tal = lds + ring_region_words();
+  uint32_t* const fres = out.fresnel_bounds ? tal + ((out.tally.words + 3u) & ~3u) : nullptr;
   zero_lds(tal, out.tally.words);
+  if (fres) fill_fresnel_bounds(P, fres, fresnel_tables(kSurf));
   __syncthreads();
   const uint32_t lane = lane_id();
   WaveCounters<kCount> wc;
@@ -1316,7 +1321,7 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
     StepInfo info{};
     bool finished = false;
     if (alive) {
-      const bool still = step<kSurf>(P, rng, ph, fate, info);
+      const bool still = step<kSurf>(P, rng, ph, fate, info, nullptr, fres);
       ++done;
       if (!still || done >= P.max_steps) {
         if (still) fate = kMaxSteps;
@@ -2059,7 +2064,7 @@ void launch_gensteps_t(const DevParams& P, PhiloxKey key, uint64_t first_id,
                        const WavefrontShape& s, const GenstepArgs& g, unsigned long long* queue,
                        const LaunchOut& out, hipStream_t stream) {
   hipLaunchKernelGGL((genstep_kernel<R, C, S>), dim3(s.grid), dim3(s.block),
-                     static_cast<size_t>(out.tally.words) * sizeof(uint32_t), stream, P, key,
+                     history_lds_bytes(out.tally.words, out.fresnel_bounds, S), stream, P, key,
                      first_id, s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g, queue,
                      out);
 }
@@ -2070,8 +2075,9 @@ void launch_gensteps_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_id
                             unsigned long long* queue, const LaunchOut& out,
                             hipStream_t stream) {
   hipLaunchKernelGGL((genstep_ring_kernel<C, S>), dim3(s.grid), dim3(s.block),
-                     ring_lds_bytes(out.tally.words, s.block), stream, P, key, first_id,
-                     s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g, queue, out);
+                     wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds),
+                     stream, P, key, first_id, s.steps_per_refill, s.queue_chunk,
+                     s.slots_per_claim, g, queue, out);
 }
 
 template <bool R, bool C>
@@ -2197,12 +2203,15 @@ void launch_gensteps(const DevParams& P, PhiloxKey key, uint64_t first_id,
     launch_gensteps_surf<false, false>(P, key, first_id, shape, g, queue, out, stream);
 }
 
-int genstep_blocks_per_cu(bool records, int block, size_t lds, bool ring) {
+int genstep_blocks_per_cu(bool records, int block, size_t lds, bool ring, int surface,
+                          bool bounds) {
+  const uint32_t words = static_cast<uint32_t>(lds / sizeof(uint32_t));
   if (!records && ring)
     return blocks_per_cu(genstep_ring_kernel<false, kLambertian>, block,
-                         ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
-  return records ? blocks_per_cu(genstep_kernel<true, false, kLambertian>, block, lds)
-                 : blocks_per_cu(genstep_kernel<false, false, kLambertian>, block, lds);
+                         wavefront_ring_lds_bytes(words, block, surface, bounds));
+  const size_t all = history_lds_bytes(words, bounds, surface);
+  return records ? blocks_per_cu(genstep_kernel<true, false, kLambertian>, block, all)
+                 : blocks_per_cu(genstep_kernel<false, false, kLambertian>, block, all);
 }
 
 int batch_blocks_per_cu(int block, uint32_t n_events, uint32_t words, bool ring) {
