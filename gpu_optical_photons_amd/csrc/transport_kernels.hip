@@ -552,7 +552,25 @@ __global__ __launch_bounds__(1024) void wavefront_next_kernel(DevParams P0, Phil
 // next refill point, once per `steps_per_refill` steps instead of every step.
 // Per photon the words drawn and the arithmetic are those of the other
 // kernels.
-template <bool kCount, int kSurf>
+// A run's common case, fixed at compile time for the kernels instantiated
+// with it: the short divisions in range (fast_div == 3), neither interface
+// index-matched, and the Rayleigh table present. The compiler then drops the
+// IEEE-division and matched-interface paths and the scalar registers that
+// select them (fewer spills of loop-invariant values). The host launches such
+// an instantiation only when common_case() holds.
+__device__ __forceinline__ void assume_common_case(const DevParams& P) {
+  __builtin_assume(P.rayleigh != nullptr);
+  __builtin_assume(P.fast_div == 3u);
+  __builtin_assume(P.det_matched == 0u);
+  __builtin_assume(P.out_matched == 0u);
+}
+
+bool common_case(const DevParams& P) {
+  return P.rayleigh != nullptr && P.fast_div == 3u && P.det_matched == 0u &&
+         P.out_matched == 0u;
+}
+
+template <bool kCount, int kSurf, bool kCommon>
 __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, PhiloxKey key,
                                                               uint64_t first_id, uint64_t n,
                                                               uint32_t steps_per_refill,
@@ -560,6 +578,7 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
                                                               unsigned long long* queue,
                                                               LaunchOut out) {
   const DevParams& P = P0;
+  if (kCommon) assume_common_case(P0);
   extern __shared__ uint32_t lds[];
   // LDS: the ring region first (at address 0, a multiple of its size), then
   // the block's tally, then the Fresnel cell bounds if kept (readout, and the
@@ -1793,10 +1812,13 @@ template <bool C, int S>
 void launch_wavefront_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n,
                              const WavefrontShape& s, unsigned long long* queue,
                              const LaunchOut& out, hipStream_t stream) {
-  hipLaunchKernelGGL((wavefront_ring_kernel<C, S>), dim3(s.grid), dim3(s.block),
-                     wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds),
-                     stream, P, key,
-                     first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+  const size_t lds = wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds);
+  if (common_case(P))
+    hipLaunchKernelGGL((wavefront_ring_kernel<C, S, true>), dim3(s.grid), dim3(s.block), lds,
+                       stream, P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+  else
+    hipLaunchKernelGGL((wavefront_ring_kernel<C, S, false>), dim3(s.grid), dim3(s.block), lds,
+                       stream, P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
 }
 
 template <bool C, int S>
@@ -1927,14 +1949,22 @@ size_t wavefront_ring_lds_bytes(uint32_t tally_words, int block, int surface, bo
 
 bool ring_kernels_start_at_lds_zero() {
   const void* kernels[] = {
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kPolished>),
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kBlack>),
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kSpecular>),
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kLambertian>),
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kPolished>),
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kBlack>),
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kSpecular>),
-      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kLambertian>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kPolished, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kBlack, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kSpecular, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kLambertian, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kPolished, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kBlack, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kSpecular, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kLambertian, false>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kPolished, true>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kBlack, true>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kSpecular, true>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<false, kLambertian, true>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kPolished, true>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kBlack, true>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kSpecular, true>),
+      reinterpret_cast<const void*>(&wavefront_ring_kernel<true, kLambertian, true>),
       reinterpret_cast<const void*>(&batch_ring_kernel<false, false>),
       reinterpret_cast<const void*>(&batch_ring_kernel<false, true>),
       reinterpret_cast<const void*>(&batch_ring_kernel<true, false>),
@@ -2133,7 +2163,7 @@ int wavefront_blocks_per_cu(bool records, bool tally, int block, size_t lds, boo
     return blocks_per_cu(wavefront_phase_kernel<false, kLambertian>, block,
                          phase_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)), block));
   if (ring)
-    return blocks_per_cu(wavefront_ring_kernel<false, kLambertian>, block,
+    return blocks_per_cu(wavefront_ring_kernel<false, kLambertian, true>, block,
                          wavefront_ring_lds_bytes(static_cast<uint32_t>(lds / sizeof(uint32_t)),
                                                   block, surface, bounds));
   if (successor) return blocks_per_cu(wavefront_next_kernel<false, kLambertian>, block, lds);
