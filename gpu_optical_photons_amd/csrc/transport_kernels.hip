@@ -252,13 +252,32 @@ __device__ __forceinline__ void fill_fresnel_bounds(const DevParams& P, uint32_t
   }
 }
 
+// A run's common case, fixed at compile time for the kernels instantiated
+// with it: the short divisions in range (fast_div == 3), neither interface
+// index-matched, and the Rayleigh table present. The compiler then drops the
+// IEEE-division and matched-interface paths and the scalar registers that
+// select them (fewer spills of loop-invariant values). The host launches such
+// an instantiation only when common_case() holds.
+__device__ __forceinline__ void assume_common_case(const DevParams& P) {
+  __builtin_assume(P.rayleigh != nullptr);
+  __builtin_assume(P.fast_div == 3u);
+  __builtin_assume(P.det_matched == 0u);
+  __builtin_assume(P.out_matched == 0u);
+}
+
+bool common_case(const DevParams& P) {
+  return P.rayleigh != nullptr && P.fast_div == 3u && P.det_matched == 0u &&
+         P.out_matched == 0u;
+}
+
 // ---------------------------------------------------------------------------
 // History-based kernel. Records: the grid covers n, lane i <-> photon i.
 // Tally: waves stride over the photons in steps of the whole grid.
-template <bool kRec, bool kTally, bool kEvt, bool kCount>
+template <bool kRec, bool kTally, bool kEvt, bool kCount, bool kCommon = false>
 __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey key,
                                                        uint64_t first_id, uint64_t n,
                                                        LaunchOut out) {
+  if (kCommon) assume_common_case(P);
   extern __shared__ uint32_t lds[];
   // LDS: the tally (if any), then the Fresnel cell bounds (when the launch
   // keeps them; never for the short per-event launches of a batch)
@@ -311,13 +330,14 @@ __global__ __launch_bounds__(1024) void history_kernel(DevParams P, PhiloxKey ke
 
 // ---------------------------------------------------------------------------
 // Persistent wavefront kernel (see the file comment).
-template <bool kRec, bool kTally, bool kCount, int kSurf = -1>
+template <bool kRec, bool kTally, bool kCount, int kSurf = -1, bool kCommon = false>
 __global__ __launch_bounds__(1024) void wavefront_kernel(DevParams P0, PhiloxKey key,
                                                          uint64_t first_id, uint64_t n,
                                                          uint32_t steps_per_refill, uint32_t chunk,
                                                          unsigned long long* queue,
                                                          LaunchOut out) {
   const DevParams& P = P0;
+  if (kCommon) assume_common_case(P0);
   extern __shared__ uint32_t lds[];
   // LDS: the tally (if any), then the Fresnel cell bounds (if kept)
   uint32_t* const fres =
@@ -552,24 +572,6 @@ __global__ __launch_bounds__(1024) void wavefront_next_kernel(DevParams P0, Phil
 // next refill point, once per `steps_per_refill` steps instead of every step.
 // Per photon the words drawn and the arithmetic are those of the other
 // kernels.
-// A run's common case, fixed at compile time for the kernels instantiated
-// with it: the short divisions in range (fast_div == 3), neither interface
-// index-matched, and the Rayleigh table present. The compiler then drops the
-// IEEE-division and matched-interface paths and the scalar registers that
-// select them (fewer spills of loop-invariant values). The host launches such
-// an instantiation only when common_case() holds.
-__device__ __forceinline__ void assume_common_case(const DevParams& P) {
-  __builtin_assume(P.rayleigh != nullptr);
-  __builtin_assume(P.fast_div == 3u);
-  __builtin_assume(P.det_matched == 0u);
-  __builtin_assume(P.out_matched == 0u);
-}
-
-bool common_case(const DevParams& P) {
-  return P.rayleigh != nullptr && P.fast_div == 3u && P.det_matched == 0u &&
-         P.out_matched == 0u;
-}
-
 template <bool kCount, int kSurf, bool kCommon>
 __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, PhiloxKey key,
                                                               uint64_t first_id, uint64_t n,
@@ -1114,7 +1116,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
 // (slots claimed `per_claim` per atomic, each slot one chunk of one
 // genstep); the geometry and tally are the run's (LDS histograms, as
 // wavefront_kernel), and each lane carries its genstep's segment.
-template <bool kRec, bool kCount, int kSurf>
+template <bool kRec, bool kCount, int kSurf, bool kCommon = false>
 __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey key,
                                                        uint64_t first_id,
                                                        uint32_t steps_per_refill, uint32_t chunk,
@@ -1122,6 +1124,7 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
                                                        unsigned long long* queue,
                                                        LaunchOut out) {
   const DevParams& P = P0;
+  if (kCommon) assume_common_case(P0);
   extern __shared__ uint32_t lds[];
   // LDS: the tally, then the Fresnel cell bounds (if kept)
   uint32_t* const fres = out.fresnel_bounds ? lds + ((out.tally.words + 3u) & ~3u) : nullptr;
@@ -1225,7 +1228,7 @@ __global__ __launch_bounds__(1024) void genstep_kernel(DevParams P0, PhiloxKey k
 // passes and deferred tallying of wavefront_ring_kernel. A lane keeps the
 // index of its genstep from the refill point to its emission, which draws
 // from the ring like the step.
-template <bool kCount, int kSurf>
+template <bool kCount, int kSurf, bool kCommon = false>
 __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, PhiloxKey key,
                                                             uint64_t first_id,
                                                             uint32_t steps_per_refill,
@@ -1234,6 +1237,7 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
                                                             unsigned long long* queue,
                                                             LaunchOut out) {
   const DevParams& P = P0;
+  if (kCommon) assume_common_case(P0);
   extern __shared__ uint32_t lds[];
   // LDS: the ring region first (at address 0, a multiple of its size), then
   // the block's tally, then the Fresnel cell bounds if kept
@@ -1804,8 +1808,12 @@ void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint
                       int grid, const LaunchOut& out, hipStream_t stream) {
   const size_t lds = history_lds_bytes(T ? out.tally.words : 0u, !E && out.fresnel_bounds,
                                        static_cast<int>(P.surface));
-  hipLaunchKernelGGL((history_kernel<R, T, E, C>), dim3(grid), dim3(block), lds, stream, P, key,
-                     first_id, n, out);
+  if (!E && common_case(P))
+    hipLaunchKernelGGL((history_kernel<R, T, E, C, true>), dim3(grid), dim3(block), lds, stream,
+                       P, key, first_id, n, out);
+  else
+    hipLaunchKernelGGL((history_kernel<R, T, E, C>), dim3(grid), dim3(block), lds, stream, P, key,
+                       first_id, n, out);
 }
 
 template <bool C, int S>
@@ -1845,8 +1853,12 @@ void launch_wavefront_t(const DevParams& P, PhiloxKey key, uint64_t first_id, ui
                         hipStream_t stream) {
   const size_t lds = history_lds_bytes(T ? out.tally.words : 0u, out.fresnel_bounds,
                                        S >= 0 ? S : static_cast<int>(P.surface));
-  hipLaunchKernelGGL((wavefront_kernel<R, T, C, S>), dim3(s.grid), dim3(s.block), lds, stream,
-                     P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+  if (common_case(P))
+    hipLaunchKernelGGL((wavefront_kernel<R, T, C, S, true>), dim3(s.grid), dim3(s.block), lds,
+                       stream, P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
+  else
+    hipLaunchKernelGGL((wavefront_kernel<R, T, C, S>), dim3(s.grid), dim3(s.block), lds, stream,
+                       P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
 }
 
 // Tally launches specialised on the run's surface model.
@@ -1977,6 +1989,14 @@ bool ring_kernels_start_at_lds_zero() {
       reinterpret_cast<const void*>(&genstep_ring_kernel<true, kBlack>),
       reinterpret_cast<const void*>(&genstep_ring_kernel<true, kSpecular>),
       reinterpret_cast<const void*>(&genstep_ring_kernel<true, kLambertian>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kPolished, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kBlack, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kSpecular, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<false, kLambertian, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kPolished, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kBlack, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kSpecular, true>),
+      reinterpret_cast<const void*>(&genstep_ring_kernel<true, kLambertian, true>),
   };
   for (const void* k : kernels) {
     hipFuncAttributes a{};
@@ -2093,10 +2113,15 @@ template <bool R, bool C, int S>
 void launch_gensteps_t(const DevParams& P, PhiloxKey key, uint64_t first_id,
                        const WavefrontShape& s, const GenstepArgs& g, unsigned long long* queue,
                        const LaunchOut& out, hipStream_t stream) {
-  hipLaunchKernelGGL((genstep_kernel<R, C, S>), dim3(s.grid), dim3(s.block),
-                     history_lds_bytes(out.tally.words, out.fresnel_bounds, S), stream, P, key,
-                     first_id, s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g, queue,
-                     out);
+  const size_t lds = history_lds_bytes(out.tally.words, out.fresnel_bounds, S);
+  if (common_case(P))
+    hipLaunchKernelGGL((genstep_kernel<R, C, S, true>), dim3(s.grid), dim3(s.block), lds, stream,
+                       P, key, first_id, s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g,
+                       queue, out);
+  else
+    hipLaunchKernelGGL((genstep_kernel<R, C, S>), dim3(s.grid), dim3(s.block), lds, stream, P,
+                       key, first_id, s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g,
+                       queue, out);
 }
 
 template <bool C, int S>
@@ -2104,10 +2129,15 @@ void launch_gensteps_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_id
                             const WavefrontShape& s, const GenstepArgs& g,
                             unsigned long long* queue, const LaunchOut& out,
                             hipStream_t stream) {
-  hipLaunchKernelGGL((genstep_ring_kernel<C, S>), dim3(s.grid), dim3(s.block),
-                     wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds),
-                     stream, P, key, first_id, s.steps_per_refill, s.queue_chunk,
-                     s.slots_per_claim, g, queue, out);
+  const size_t lds = wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds);
+  if (common_case(P))
+    hipLaunchKernelGGL((genstep_ring_kernel<C, S, true>), dim3(s.grid), dim3(s.block), lds,
+                       stream, P, key, first_id, s.steps_per_refill, s.queue_chunk,
+                       s.slots_per_claim, g, queue, out);
+  else
+    hipLaunchKernelGGL((genstep_ring_kernel<C, S>), dim3(s.grid), dim3(s.block), lds, stream, P,
+                       key, first_id, s.steps_per_refill, s.queue_chunk, s.slots_per_claim, g,
+                       queue, out);
 }
 
 template <bool R, bool C>
