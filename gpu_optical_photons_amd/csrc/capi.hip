@@ -195,6 +195,7 @@ struct Context {
   // group of iterations; the pinned status words the host polls.
   DevBuf sorted;
   PinnedBuf sorted_status;
+  PinnedBuf tally_host;  // pinned landing buffer of a tally's device-to-host copy
   hipGraphExec_t sorted_exec = nullptr;
   std::string sorted_key;
   // Device address of noalloc_store::scratch on each device, looked up at
@@ -443,8 +444,10 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
     throw std::runtime_error("too many photons for one tally launch; split the run");
   unsigned long long* scratch = ensure_scratch(c, grid, tm.alloc_ms);
   const auto ta = std::chrono::steady_clock::now();
+  const size_t tally_bytes = (static_cast<size_t>(T.words) + 2) * 8;
   bool grew = c.partials.ensure(static_cast<size_t>(grid) * T.words * sizeof(uint32_t));
-  grew |= c.tally.ensure((static_cast<size_t>(T.words) + 2) * 8);
+  grew |= c.tally.ensure(tally_bytes);
+  grew |= c.tally_host.ensure(tally_bytes);
   if (grew) tm.alloc_ms += ms_since(ta);
 
   LaunchOut out{};
@@ -458,8 +461,9 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
   out.fresnel_bounds = wavefront ? shape.fresnel_bounds
                                  : history_bounds(c, T.words, static_cast<int>(p.surface));
 
-  launch_zero_u64(scratch, kScratchSums + 2 * static_cast<size_t>(grid), c.stream);
-  launch_zero_u64(c.tally.as<unsigned long long>(), static_cast<size_t>(T.words) + 2, c.stream);
+  launch_zero_u64_pair(scratch, kScratchSums + 2 * static_cast<size_t>(grid),
+                       c.tally.as<unsigned long long>(), static_cast<size_t>(T.words) + 2,
+                       c.stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.ev[0], c.stream));
   if (wavefront)
@@ -472,14 +476,16 @@ void run_tally_common(Context& c, const optphot_sim_params& p, const optphot_tal
   launch_tally_reduce(T, grid, out.partials, out.sums, c.tally.as<unsigned long long>(), c.stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c.ev[2], c.stream));
+  // the copy is queued behind the reduction, so the host waits once; it
+  // lands in pinned memory (a pageable destination costs a staged copy)
+  HIPCHK(hipMemcpyAsync(c.tally_host.ptr, c.tally.ptr, tally_bytes, hipMemcpyDeviceToHost,
+                        c.stream));
+  HIPCHK(hipEventRecord(c.ev[3], c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
+  std::memcpy(tally_out, c.tally_host.ptr, tally_bytes);
   const float k_ms = c.elapsed(0, 1);
   tm.kernel_ms = c.elapsed(0, 2);
-
-  const auto td = std::chrono::steady_clock::now();
-  HIPCHK(hipMemcpyAsync(tally_out, c.tally.ptr, (static_cast<size_t>(T.words) + 2) * 8,
-                        hipMemcpyDeviceToHost, c.stream));
-  HIPCHK(hipStreamSynchronize(c.stream));
-  tm.d2h_ms = ms_since(td);
+  tm.d2h_ms = c.elapsed(2, 3);
   read_counters(c, counters);
   if (stats) {  // counted by the same launch (out.counters is set when stats is)
     optphot_step_counters cnt{};

@@ -245,6 +245,9 @@ void launch_tally_reduce(const DevTally& t, int blocks, const uint32_t* partials
 
 // Fills n u64 words with zero (a kernel, so it may sit inside captured graphs).
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream);
+// The same for two ranges, in one launch.
+void launch_zero_u64_pair(unsigned long long* a, uint64_t na, unsigned long long* b, uint64_t nb,
+                          hipStream_t stream);
 
 // Fills the 2^24-entry Rayleigh-cosine table (DevParams::rayleigh).
 constexpr size_t kRayleighTableEntries = size_t{1} << 24;

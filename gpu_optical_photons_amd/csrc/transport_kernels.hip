@@ -1723,6 +1723,19 @@ __global__ void rayleigh_table_kernel(float* table) {
     table[k] = rayleigh_cos(to_unit(k << 8));
 }
 
+// Clears two ranges in one launch (a tally run's scratch and its tally).
+__global__ void zero_u64_pair_kernel(unsigned long long* a, uint64_t na, unsigned long long* b,
+                                     uint64_t nb) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < na + nb;
+       i += stride) {
+    if (i < na)
+      a[i] = 0ull;
+    else
+      b[i - na] = 0ull;
+  }
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p, uint64_t n) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -2169,6 +2182,15 @@ void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
 
 void launch_rayleigh_table(float* table, hipStream_t stream) {
   hipLaunchKernelGGL(rayleigh_table_kernel, dim3(4096), dim3(256), 0, stream, table);
+}
+
+void launch_zero_u64_pair(unsigned long long* a, uint64_t na, unsigned long long* b, uint64_t nb,
+                          hipStream_t stream) {
+  if (na + nb == 0) return;
+  const int threads = 256;
+  const uint64_t want = (na + nb + threads - 1) / threads;
+  const int grid = static_cast<int>(want < 1024 ? want : 1024);
+  hipLaunchKernelGGL(zero_u64_pair_kernel, dim3(grid), dim3(threads), 0, stream, a, na, b, nb);
 }
 
 void launch_zero_u64(unsigned long long* p, uint64_t n, hipStream_t stream) {
