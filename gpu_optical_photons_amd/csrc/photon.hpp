@@ -285,14 +285,17 @@ __device__ __forceinline__ float sqrt_n(float x) {
 // portable math, pmath.hpp:62-130): bit-identical to the CPU build with
 // OPTPHOT_PORTABLE_MATH.
 
+// x positive and normal. The reference splits x = m 2^e with m the mantissa
+// in [1, 2), then halves m and increments e when m > 1.41421356f, i.e. when
+// the mantissa field is at least 0x3504f4 (1.41421356f rounds to 0x3fb504f3).
+// With d = bits(x) - 0x3f3504f4 (0x3f3504f4 = 0x3fb504f4 halved) the borrow
+// out of the mantissa field makes e = d >> 23 (arithmetic) and
+// bits(m) = (d & 0x7fffff) + 0x3f3504f4 exactly those: 4 integer operations
+// instead of the extraction, compare and two selects.
 __device__ __forceinline__ float p_log(float x) {
-  const uint32_t bits = __float_as_uint(x);
-  int e = static_cast<int>((bits >> 23) & 0xffu) - 127;
-  float m = __uint_as_float((bits & 0x007fffffu) | 0x3f800000u);
-  if (m > 1.41421356f) {
-    m = 0.5f * m;
-    ++e;
-  }
+  const int d = static_cast<int>(__float_as_uint(x)) - 0x3f3504f4;
+  const int e = d >> 23;
+  const float m = __int_as_float((d & 0x007fffff) + 0x3f3504f4);
   // m in [0.707, 1.415): m - 1 is 0 or at least 2^-24 in magnitude, m + 1 in [1.7, 2.5]
   const float s = div_n(m - 1.0f, m + 1.0f);
   const float s2 = s * s;
@@ -303,15 +306,20 @@ __device__ __forceinline__ float p_log(float x) {
   return fe * 6.9313812256e-01f + (2.0f * s * poly + fe * 9.0580006145e-06f);
 }
 
-__device__ __forceinline__ void p_sincos_turns(float u, float& sn, float& cs) {
-  const float v = 4.0f * u;
-  float q = __builtin_floorf(v);
-  float r = v - q;
-  if (r > 0.5f) {
-    r -= 1.0f;
-    q += 1.0f;
-  }
-  const float x = r * 1.57079632679489662f;
+// sin and cos of 2 pi u for the uniform u = (k + 1) 2^-24 drawn from word w
+// (k = w >> 8, to_unit). The reference reduces v = 4 u: q = floor(v),
+// r = v - q, and r > 1/2 becomes r - 1 with q + 1; then x = r (pi / 2). Every
+// step before the product is exact: v = (k + 1) 2^-22, so q and r are the
+// quotient and remainder of k + 1 by 2^22, rounded to nearest with ties down
+// (r = 1/2 stays). With t = k + 2^21: q = t >> 22 and r = ((t & (2^22 - 1))
+// - (2^21 - 1)) 2^-22, an integer in (-2^21, 2^21] times 2^-22, and
+// x = r' (2^-22 pi/2) rounds as r (pi/2) does (a power-of-two scale commutes
+// with rounding, and x is 0 or normal). The polynomials are unchanged.
+__device__ __forceinline__ void p_sincos_word(uint32_t w, float& sn, float& cs) {
+  const uint32_t t = (w >> 8) + (1u << 21);
+  const uint32_t q = t >> 22;
+  const int ri = static_cast<int>(t & 0x003fffffu) - ((1 << 21) - 1);
+  const float x = static_cast<float>(ri) * (1.57079632679489662f * 0x1p-22f);
   const float x2 = x * x;
   const float sp =
       x * (1.0f + x2 * (-1.0f / 6.0f +
@@ -320,7 +328,7 @@ __device__ __forceinline__ void p_sincos_turns(float u, float& sn, float& cs) {
       1.0f + x2 * (-0.5f + x2 * (1.0f / 24.0f +
                                  x2 * (-1.0f / 720.0f +
                                        x2 * (1.0f / 40320.0f + x2 * (-1.0f / 3628800.0f)))));
-  const int quadrant = static_cast<int>(q) & 3;
+  const int quadrant = static_cast<int>(q & 3u);
   // quadrant rotation as selects: (s, c) = (sp, cp), (cp, -sp), (-sp, -cp), (-cp, sp)
   const bool odd = quadrant & 1;
   const float a = odd ? cp : sp;
@@ -365,8 +373,9 @@ __device__ __forceinline__ float inv_sqrt_near_one(float s) {
 }
 
 // rotate_about (vec3.hpp:35-53): Duff et al. basis around `n`, then
-// normalize((st cp) t1 + (st sp) t2 + ct n) with normalize(a) = (1/|a|) a.
-__device__ __forceinline__ V3 rotate_about(V3 n, float cos_theta, float phi_turns) {
+// normalize((st cp) t1 + (st sp) t2 + ct n) with normalize(a) = (1/|a|) a,
+// the azimuth 2 pi to_unit(phi_word).
+__device__ __forceinline__ V3 rotate_about(V3 n, float cos_theta, uint32_t phi_word) {
   const float sign = __builtin_copysignf(1.0f, n.z);
   // n is a unit vector and sign = copysign(1, n.z): sign + n.z in [1, 2] or [-2, -1]
   const float a = div_n(-1.0f, sign + n.z);
@@ -376,7 +385,7 @@ __device__ __forceinline__ V3 rotate_about(V3 n, float cos_theta, float phi_turn
   // |cos| <= 1: 1 - cos^2 is 0 or at least 2^-24
   const float st = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_theta * cos_theta));
   float sp, cp;
-  p_sincos_turns(phi_turns, sp, cp);
+  p_sincos_word(phi_word, sp, cp);
   const float c1 = st * cp, c2 = st * sp;
   const V3 d{(c1 * t1.x + c2 * t2.x) + cos_theta * n.x, (c1 * t1.y + c2 * t2.y) + cos_theta * n.y,
              (c1 * t1.z + c2 * t2.z) + cos_theta * n.z};
@@ -537,13 +546,13 @@ __device__ __forceinline__ void emit(const DevParams& P, RNG& rng, Photon& ph) {
   // through peek with pos advanced first.
   rng.pos += o;
   const float ua = to_unit(rng.template peek<0>());
-  const float ub = to_unit(rng.template peek<1>());
+  const uint32_t wb = rng.template peek<1>();
   const float ut = to_unit(rng.template peek<2>());
   rng.pos += 3u;
   const float cos_t = 2.0f * ua - 1.0f;
   const float sin_t = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));  // 0 or >= 2^-24
   float sp, cp;
-  p_sincos_turns(ub, sp, cp);
+  p_sincos_word(wb, sp, cp);
   ph.d = V3{sin_t * cp, sin_t * sp, cos_t};
   ph.t = P.tau > 0.0f ? -P.tau * p_log(ut) : 0.0f;
   ph.path = 0.0f;
@@ -577,12 +586,12 @@ __device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenste
                                              Photon& ph) {
   if (G.cone_cos > 0.0f) {
     const float u = to_unit(rng.template peek<0>());
-    const float up = to_unit(rng.template peek<1>());
+    const uint32_t wp = rng.template peek<1>();
     rng.pos += 2u;
     const float dx = G.x1 - G.x0, dy = G.y1 - G.y0, dz = G.z1 - G.z0;
     ph.p = V3{G.x0 + u * dx, G.y0 + u * dy, G.z0 + u * dz};
     const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-    ph.d = rotate_about(V3{inv * dx, inv * dy, inv * dz}, G.cone_cos, up);
+    ph.d = rotate_about(V3{inv * dx, inv * dy, inv * dz}, G.cone_cos, wp);
     ph.t = G.t0 + u * (G.t1 - G.t0);
     ph.path = 0.0f;
     ph.nb = 0u;
@@ -598,13 +607,13 @@ __device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenste
     rng.pos += 1u;
   }
   const float ua = to_unit(rng.template peek<0>());
-  const float ub = to_unit(rng.template peek<1>());
+  const uint32_t wb = rng.template peek<1>();
   const float ut = to_unit(rng.template peek<2>());
   rng.pos += 3u;
   const float cos_t = 2.0f * ua - 1.0f;
   const float sin_t = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));  // 0 or >= 2^-24
   float sp, cp;
-  p_sincos_turns(ub, sp, cp);
+  p_sincos_word(wb, sp, cp);
   ph.d = V3{sin_t * cp, sin_t * sp, cos_t};
   ph.t = t0 + (P.tau > 0.0f ? -P.tau * p_log(ut) : 0.0f);
   ph.path = 0.0f;
@@ -625,6 +634,16 @@ __device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenste
 __device__ __forceinline__ float box_div(const DevParams& P, float num, float d) {
   if ((P.fast_div & 1u) && __builtin_fabsf(d) >= 0x1p-60f) return div_n(num, d);
   return num / d;
+}
+
+// fminf(fmaxf(v, -h), h), the clamp of the face snap (geometry.hpp:57-63).
+// With the box half sizes in [2^-60, 2^40] (P.fast_div bit 0) every position
+// is finite (each advance is bounded by the distance to a wall) and h > 0, so
+// the median of v, -h and h is the same value in one instruction (no NaN
+// operand, and no tie between a zero and a signed zero).
+__device__ __forceinline__ float clamp_half(const DevParams& P, float v, float h) {
+  if (P.fast_div & 1u) return __builtin_amdgcn_fmed3f(v, -h, h);
+  return __builtin_fminf(__builtin_fmaxf(v, -h), h);
 }
 
 // Where the step ends: absorbed in the bulk, scattered, or on face `face`
@@ -695,9 +714,9 @@ __device__ __forceinline__ Advance advance(const DevParams& P, const RNG& rng, P
   a.plus = face & 1;
   if (a.boundary) {  // snap onto the face plane, clamp the others
     const float s = a.plus ? 1.0f : -1.0f;
-    ph.p.x = a.axis == 0 ? s * P.hx : __builtin_fminf(__builtin_fmaxf(ph.p.x, -P.hx), P.hx);
-    ph.p.y = a.axis == 1 ? s * P.hy : __builtin_fminf(__builtin_fmaxf(ph.p.y, -P.hy), P.hy);
-    ph.p.z = a.axis == 2 ? s * P.hz : __builtin_fminf(__builtin_fmaxf(ph.p.z, -P.hz), P.hz);
+    ph.p.x = a.axis == 0 ? s * P.hx : clamp_half(P, ph.p.x, P.hx);
+    ph.p.y = a.axis == 1 ? s * P.hy : clamp_half(P, ph.p.y, P.hy);
+    ph.p.z = a.axis == 2 ? s * P.hz : clamp_half(P, ph.p.z, P.hz);
     ++ph.nb;
   }
   if (a.scatter) ++ph.ns;
@@ -827,9 +846,9 @@ __device__ __forceinline__ uint32_t fresnel_cell_bounds(float n1, float n2, floa
 // Specular reflection on an axis-aligned face: d - 2 (d.n) n rounds to
 // exactly the negation of the approached component.
 __device__ __forceinline__ void flip_axis(V3& d, int axis) {
-  if (axis == 0) d.x = -d.x;
-  if (axis == 1) d.y = -d.y;
-  if (axis == 2) d.z = -d.z;
+  d.x = axis == 0 ? -d.x : d.x;
+  d.y = axis == 1 ? -d.y : d.y;
+  d.z = axis == 2 ? -d.z : d.z;
 }
 
 // Phase 3: the new direction after Rayleigh scattering (about the direction,
@@ -838,7 +857,6 @@ __device__ __forceinline__ void flip_axis(V3& d, int axis) {
 // from the two words wc (cos) and wp (azimuth).
 __device__ __forceinline__ V3 redirect(const DevParams& P, bool scatter, V3 d, int axis, bool plus,
                                        uint32_t wc, uint32_t wp) {
-  const float up = to_unit(wp);
   float cos_theta;
   if (scatter)
     cos_theta = rayleigh_cos_word(P.rayleigh, wc);
@@ -847,7 +865,7 @@ __device__ __forceinline__ V3 redirect(const DevParams& P, bool scatter, V3 d, i
   const float s = plus ? 1.0f : -1.0f;
   // -n keeps the reference's signed zeros: -(+0) = -0 off-axis.
   const V3 inward{axis == 0 ? -s : -0.0f, axis == 1 ? -s : -0.0f, axis == 2 ? -s : -0.0f};
-  return rotate_about(scatter ? d : inward, cos_theta, up);
+  return rotate_about(scatter ? d : inward, cos_theta, wp);
 }
 
 // One transport step (transport.hpp:71-145). Returns true while the photon
