@@ -655,6 +655,17 @@ struct Advance {
   float cos_i;
 };
 
+// Some component of d is nonzero and below 2^-60 in magnitude (subnormals
+// included): (bits << 1) - 2 drops the sign and sends +-0 to the top.
+__device__ __forceinline__ bool tiny_component(float dx, float dy, float dz) {
+  constexpr uint32_t kMin = (0x21800000u << 1) - 2u;  // 2^-60
+  const uint32_t x = (__float_as_uint(dx) << 1) - 2u;
+  const uint32_t y = (__float_as_uint(dy) << 1) - 2u;
+  const uint32_t z = (__float_as_uint(dz) << 1) - 2u;
+  return (x < y ? (x < z ? x : z) : (y < z ? y : z)) < kMin;
+}
+
+
 // Phase 1: the two exponential distances (draws 0 and 1, always drawn,
 // transport.hpp:76-77), the distance to the wall approached (geometry.hpp:
 // 25-46), which event wins, and one shared advance with the snap onto a face
@@ -670,31 +681,57 @@ __device__ __forceinline__ Advance advance(const DevParams& P, const RNG& rng, P
   // --- distance to the wall approached, ties to the lower axis --------------
   float dist = kInf;
   int face = -1;
-  if (ph.d.x != 0.0f) {
-    const bool pos = ph.d.x > 0.0f;
-    const float num = (pos ? P.hx : -P.hx) - ph.p.x;
-    const float tt = box_div(P, num, ph.d.x);
-    if (tt < dist) {
-      dist = tt;
-      face = pos ? 1 : 0;
+  // (evaluated ahead of the branch: inside its condition, hipcc 7.2 keeps a
+  // stack copy of the direction and reads cos_i below from scratch memory)
+  const bool tiny = tiny_component(ph.d.x, ph.d.y, ph.d.z);
+  if ((P.fast_div & 1u) && !tiny) {
+    // Every component is 0 or at least 2^-60 in magnitude. div_n gives the
+    // IEEE quotient of a nonzero one (box_div), and NaN for a zero one (the
+    // reciprocal estimate of 0 is infinite, and 0 * inf is NaN). NaN loses
+    // every comparison, so a zero component leaves dist and face as the
+    // skipped axis of the general path does, without a branch per axis.
+    const float tx = div_n((ph.d.x > 0.0f ? P.hx : -P.hx) - ph.p.x, ph.d.x);
+    const float ty = div_n((ph.d.y > 0.0f ? P.hy : -P.hy) - ph.p.y, ph.d.y);
+    const float tz = div_n((ph.d.z > 0.0f ? P.hz : -P.hz) - ph.p.z, ph.d.z);
+    if (tx < dist) {
+      dist = tx;
+      face = ph.d.x > 0.0f ? 1 : 0;
     }
-  }
-  if (ph.d.y != 0.0f) {
-    const bool pos = ph.d.y > 0.0f;
-    const float num = (pos ? P.hy : -P.hy) - ph.p.y;
-    const float tt = box_div(P, num, ph.d.y);
-    if (tt < dist) {
-      dist = tt;
-      face = pos ? 3 : 2;
+    if (ty < dist) {
+      dist = ty;
+      face = ph.d.y > 0.0f ? 3 : 2;
     }
-  }
-  if (ph.d.z != 0.0f) {
-    const bool pos = ph.d.z > 0.0f;
-    const float num = (pos ? P.hz : -P.hz) - ph.p.z;
-    const float tt = box_div(P, num, ph.d.z);
-    if (tt < dist) {
-      dist = tt;
-      face = pos ? 5 : 4;
+    if (tz < dist) {
+      dist = tz;
+      face = ph.d.z > 0.0f ? 5 : 4;
+    }
+  } else {  // one axis at a time, skipping zero components; the quotients of box_div
+    if (ph.d.x != 0.0f) {
+      const bool pos = ph.d.x > 0.0f;
+      const float num = (pos ? P.hx : -P.hx) - ph.p.x;
+      const float tt = box_div(P, num, ph.d.x);
+      if (tt < dist) {
+        dist = tt;
+        face = pos ? 1 : 0;
+      }
+    }
+    if (ph.d.y != 0.0f) {
+      const bool pos = ph.d.y > 0.0f;
+      const float num = (pos ? P.hy : -P.hy) - ph.p.y;
+      const float tt = box_div(P, num, ph.d.y);
+      if (tt < dist) {
+        dist = tt;
+        face = pos ? 3 : 2;
+      }
+    }
+    if (ph.d.z != 0.0f) {
+      const bool pos = ph.d.z > 0.0f;
+      const float num = (pos ? P.hz : -P.hz) - ph.p.z;
+      const float tt = box_div(P, num, ph.d.z);
+      if (tt < dist) {
+        dist = tt;
+        face = pos ? 5 : 4;
+      }
     }
   }
   dist = __builtin_fmaxf(dist, 0.0f);

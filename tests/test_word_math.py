@@ -125,3 +125,19 @@ def test_sincos_values_equal_the_oracle(oracle_portable):
     ref = np.array([oracle_portable.sincos(F(v), portable=True) for v in u], dtype=F)
     assert np.array_equal(s.view(np.uint32), ref[:, 0].view(np.uint32))
     assert np.array_equal(c.view(np.uint32), ref[:, 1].view(np.uint32))
+
+
+def test_tiny_component_test_from_bits():
+    """(bits << 1) - 2 < (bits(2^-60) << 1) - 2 exactly for 0 < |d| < 2^-60
+    (photon.hpp: tiny_component), over zeros, subnormals, the neighbours of
+    2^-60 and a log-uniform sample of both signs."""
+    rng = np.random.default_rng(6)
+    edge = np.float32(2.0 ** -60)
+    vals = np.concatenate([
+        np.array([0.0, -0.0, 1e-45, -1e-45, 1e-39, np.inf, -np.inf, 1.0, -1.0], F),
+        np.array([np.nextafter(edge, F(0)), edge, np.nextafter(edge, F(1))], F),
+        (np.exp2(rng.uniform(-149, 0, 200_000)) * rng.choice([-1, 1], 200_000)).astype(F)])
+    b = vals.view(np.uint32).astype(np.uint64)
+    got = ((b << 1) - 2) % (1 << 32) < ((0x21800000 << 1) - 2)
+    want = (np.abs(vals) > 0) & (np.abs(vals) < edge)
+    assert np.array_equal(got, want)
