@@ -997,6 +997,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
   wr.clear(out.evt.words);
 
   uint32_t we = 0;                 // event of the wave's claimed range (uniform)
+  uint64_t we_off = 0;             // its first photon index, off[we] (uniform)
   uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
   uint64_t s_next = 0, s_end = 0;  // slots claimed by the wave, not yet opened
   bool q_done = false;
@@ -1023,6 +1024,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
         const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
         const uint64_t b1 = q_next;
         const uint32_t s1 = we;
+        const uint64_t s1_off = we_off;
         q_next += t1;
         uint64_t b2 = 0;
         uint32_t t2 = 0;
@@ -1042,10 +1044,12 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
           const unsigned long long sl = slots[c];
           const uint32_t e = static_cast<uint32_t>(sl >> 32);
           const uint64_t start = static_cast<uint64_t>(static_cast<uint32_t>(sl)) * chunk;
-          const uint64_t ne = off[e + 1] - off[e];
+          const uint64_t e_off = off[e];
+          const uint64_t ne = off[e + 1] - e_off;
           if (start >= ne) continue;
           wr.take(loc.evt, e);
           we = e;
+          we_off = e_off;
           q_next = start;
           q_end = (start + chunk < ne) ? start + chunk : ne;
           t2 = (need - t1 < q_end - q_next) ? need - t1 : static_cast<uint32_t>(q_end - q_next);
@@ -1059,7 +1063,7 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
           const uint64_t idx = r < t1 ? b1 + r : b2 + (r - t1);
           P = out.evt.table[pset];
           P.rayleigh = rtab;
-          rng.reset(first_id + off[pset] + idx);
+          rng.reset(first_id + (r < t1 ? s1_off : we_off) + idx);
           fresh = true;
         }
         wc.add(kCntEmitted, t1 + t2);
@@ -1417,6 +1421,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
   wr.clear(out.evt.words);
 
   uint32_t we = 0;                 // event of the wave's claimed range (uniform)
+  uint64_t we_off = 0;             // its first photon index, off[we] (uniform)
   uint64_t q_next = 0, q_end = 0;  // claimed, not yet started (indices within event we)
   uint64_t s_next = 0, s_end = 0;  // slots claimed by the wave, not yet opened
   bool q_done = false;
@@ -1437,6 +1442,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
       const uint32_t t1 = have < need ? static_cast<uint32_t>(have) : need;
       const uint64_t b1 = q_next;
       const uint32_t s1 = we;
+      const uint64_t s1_off = we_off;
       q_next += t1;
       uint64_t b2 = 0;
       uint32_t t2 = 0;
@@ -1456,10 +1462,12 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
         const unsigned long long sl = slots[c];
         const uint32_t e = static_cast<uint32_t>(sl >> 32);
         const uint64_t start = static_cast<uint64_t>(static_cast<uint32_t>(sl)) * chunk;
-        const uint64_t ne = off[e + 1] - off[e];
+        const uint64_t e_off = off[e];
+        const uint64_t ne = off[e + 1] - e_off;
         if (start >= ne) continue;
         wr.take(loc.evt, e);
         we = e;
+        we_off = e_off;
         q_next = start;
         q_end = (start + chunk < ne) ? start + chunk : ne;
         t2 = (need - t1 < q_end - q_next) ? need - t1 : static_cast<uint32_t>(q_end - q_next);
@@ -1474,7 +1482,7 @@ __global__ __launch_bounds__(1024) void batch_kernel(PhiloxKey key, uint64_t fir
         idx = r < t1 ? b1 + r : b2 + (r - t1);
         P = out.evt.table[pset];
         P.rayleigh = rtab;
-        rng.start(first_id + off[pset] + idx, key, 0u);
+        rng.start(first_id + (r < t1 ? s1_off : we_off) + idx, key, 0u);
         emit(P, rng, ph);
         done = 0;
         fate = kMaxSteps;
