@@ -198,8 +198,12 @@ struct RingStreamT {
 
   // `region`: kSlots << ring_shift() bytes of LDS (aligned as above when
   // kAligned); the stream starts empty.
-  __device__ __forceinline__ void place(const uint32_t* region) {
-    shift = ring_shift();
+  __device__ __forceinline__ void place(const uint32_t* region) { place(region, ring_shift()); }
+  // The same with the slot stride given (it must equal ring_shift()): a
+  // compile-time stride turns a block's four slot stores into two
+  // ds_write2st64_b32 with immediate offsets.
+  __device__ __forceinline__ void place(const uint32_t* region, uint32_t sh) {
+    shift = sh;
     mask = (kSlots - 1u) << shift;
     // kept opaque: seen through, the compiler rewrites (w << shift) & mask as
     // (w & 15) << shift, one instruction more per access

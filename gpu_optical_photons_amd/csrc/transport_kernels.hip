@@ -270,6 +270,14 @@ bool common_case(const DevParams& P) {
          P.out_matched == 0u;
 }
 
+// The ring kernels' common-case instantiations also fix the block size, so
+// the slot stride of the LDS ring is a compile-time constant: 4 bytes x 512
+// lanes, 2^11.
+constexpr int kCommonRingBlock = 512;
+constexpr uint32_t kCommonRingShift = 11u;
+static_assert((4u << 9) == (1u << kCommonRingShift) && kCommonRingBlock == (1 << 9),
+              "ring stride of the common block size");
+
 // ---------------------------------------------------------------------------
 // History-based kernel. Records: the grid covers n, lane i <-> photon i.
 // Tally: waves stride over the photons in steps of the whole grid.
@@ -602,7 +610,10 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
   bool pending = false;  // a finished photon not yet tallied
   uint32_t done = 0, fate = kMaxSteps;
   RingStream rng;
-  rng.place(lds);
+  if (kCommon)  // launched with kCommonRingBlock lanes per block only
+    rng.place(lds, kCommonRingShift);
+  else
+    rng.place(lds);
   Photon ph;
   const uint32_t emit_words = P.source == kVolume ? 6u : 3u;
 
@@ -1263,7 +1274,10 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
   uint32_t lg = 0;  // the lane's genstep, from its refill to its emission
   uint32_t done = 0, fate = kMaxSteps;
   RingStream rng;
-  rng.place(lds);
+  if (kCommon)  // launched with kCommonRingBlock lanes per block only
+    rng.place(lds, kCommonRingShift);
+  else
+    rng.place(lds);
   Photon ph;
   constexpr uint32_t kEmitWords = 4u;  // u along the segment (if any), cos, phi, delay
 
@@ -1842,7 +1856,7 @@ void launch_wavefront_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_i
                              const WavefrontShape& s, unsigned long long* queue,
                              const LaunchOut& out, hipStream_t stream) {
   const size_t lds = wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds);
-  if (common_case(P))
+  if (common_case(P) && s.block == kCommonRingBlock)
     hipLaunchKernelGGL((wavefront_ring_kernel<C, S, true>), dim3(s.grid), dim3(s.block), lds,
                        stream, P, key, first_id, n, s.steps_per_refill, s.queue_chunk, queue, out);
   else
@@ -2151,7 +2165,7 @@ void launch_gensteps_ring_t(const DevParams& P, PhiloxKey key, uint64_t first_id
                             unsigned long long* queue, const LaunchOut& out,
                             hipStream_t stream) {
   const size_t lds = wavefront_ring_lds_bytes(out.tally.words, s.block, S, out.fresnel_bounds);
-  if (common_case(P))
+  if (common_case(P) && s.block == kCommonRingBlock)
     hipLaunchKernelGGL((genstep_ring_kernel<C, S, true>), dim3(s.grid), dim3(s.block), lds,
                        stream, P, key, first_id, s.steps_per_refill, s.queue_chunk,
                        s.slots_per_claim, g, queue, out);
