@@ -663,8 +663,10 @@ __global__ __launch_bounds__(1024) void wavefront_ring_kernel(DevParams P0, Phil
 
     // ---- generation passes -------------------------------------------------
     const bool drawing = alive || fresh;
-    const uint32_t need = fresh ? emit_words + 5u : 5u;
-    while (__ballot(drawing && rng.level() < need) != 0) {
+    // 0 for a lane that does not draw: its level is never below it, so the
+    // loop's ballot is of one comparison
+    const uint32_t need = drawing ? (fresh ? emit_words + 5u : 5u) : 0u;
+    while (__ballot(rng.level() < need) != 0) {
       if (drawing && rng.room()) rng.generate(key);
     }
 
@@ -809,8 +811,10 @@ __global__ __launch_bounds__(1024) void wavefront_phase_kernel(DevParams P0, Phi
 
     // ---- generation passes and emission (per wave) ----------------------------
     const bool drawing = alive || fresh;
-    const uint32_t need = fresh ? emit_words + 5u : 5u;
-    while (__ballot(drawing && rng.level() < need) != 0) {
+    // 0 for a lane that does not draw: its level is never below it, so the
+    // loop's ballot is of one comparison
+    const uint32_t need = drawing ? (fresh ? emit_words + 5u : 5u) : 0u;
+    while (__ballot(rng.level() < need) != 0) {
       if (drawing && rng.room()) rng.generate(key);
     }
     if (fresh) {
@@ -1086,8 +1090,10 @@ __global__ __launch_bounds__(1024) void batch_ring_kernel(PhiloxKey key, uint64_
     }
 
     const bool drawing = alive || fresh;
-    const uint32_t need = fresh ? (P.source == kVolume ? 6u : 3u) + 5u : 5u;
-    while (__ballot(drawing && rng.level() < need) != 0) {
+    // 0 for a lane that does not draw: its level is never below it, so the
+    // loop's ballot is of one comparison
+    const uint32_t need = drawing ? (fresh ? (P.source == kVolume ? 6u : 3u) + 5u : 5u) : 0u;
+    while (__ballot(rng.level() < need) != 0) {
       if (drawing && rng.room()) rng.generate(key);
     }
     if (fresh) {
@@ -1340,8 +1346,10 @@ __global__ __launch_bounds__(1024) void genstep_ring_kernel(DevParams P0, Philox
 
     // ---- generation passes -------------------------------------------------
     const bool drawing = alive || fresh;
-    const uint32_t need = fresh ? kEmitWords + 5u : 5u;
-    while (__ballot(drawing && rng.level() < need) != 0) {
+    // 0 for a lane that does not draw: its level is never below it, so the
+    // loop's ballot is of one comparison
+    const uint32_t need = drawing ? (fresh ? kEmitWords + 5u : 5u) : 0u;
+    while (__ballot(rng.level() < need) != 0) {
       if (drawing && rng.room()) rng.generate(key);
     }
 
