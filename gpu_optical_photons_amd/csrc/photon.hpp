@@ -113,6 +113,15 @@ __device__ __forceinline__ float to_unit(uint32_t bits) {
   return __builtin_fmaf(static_cast<float>(bits >> 8), 1.0f / 16777216.0f, 1.0f / 16777216.0f);
 }
 
+// 2 u - 1 for u = to_unit(bits), as the reference computes it (2.0f * u -
+// 1.0f, transport.hpp:55-67): 2 u = (k + 1) 2^-23 is exact and so is
+// 2 u - 1 = (k + 1 - 2^23) 2^-23 (an integer below 2^23 in magnitude times
+// 2^-23). One FMA of k 2^-23 and 2^-23 - 1 (exact in f32) rounds that exact
+// value, so it gives the same bits.
+__device__ __forceinline__ float sym_unit(uint32_t bits) {
+  return __builtin_fmaf(static_cast<float>(bits >> 8), 0x1p-23f, 0x1p-23f - 1.0f);
+}
+
 // A photon's stream: words [4*(blk-2), 4*blk) are held in w0..w7, `pos` is
 // the next unread word inside that window. Refill keeps pos <= 3, so at least
 // five words are readable without generating.
@@ -538,10 +547,8 @@ template <class RNG>
 __device__ __forceinline__ void emit(const DevParams& P, RNG& rng, Photon& ph) {
   uint32_t o = 0;
   if (P.source == kVolume) {
-    const float u1 = to_unit(rng.template peek<0>());
-    const float u2 = to_unit(rng.template peek<1>());
-    const float u3 = to_unit(rng.template peek<2>());
-    ph.p = V3{P.hx * (2.0f * u1 - 1.0f), P.hy * (2.0f * u2 - 1.0f), P.hz * (2.0f * u3 - 1.0f)};
+    ph.p = V3{P.hx * sym_unit(rng.template peek<0>()), P.hy * sym_unit(rng.template peek<1>()),
+              P.hz * sym_unit(rng.template peek<2>())};
     o = 3;
   } else {
     ph.p = V3{P.src_x, P.src_y, P.src_z};
@@ -549,11 +556,10 @@ __device__ __forceinline__ void emit(const DevParams& P, RNG& rng, Photon& ph) {
   // The window holds words 0..7 and o <= 3, so the offsets o..o+2 are read
   // through peek with pos advanced first.
   rng.pos += o;
-  const float ua = to_unit(rng.template peek<0>());
+  const float cos_t = sym_unit(rng.template peek<0>());
   const uint32_t wb = rng.template peek<1>();
   const float ut = to_unit(rng.template peek<2>());
   rng.pos += 3u;
-  const float cos_t = 2.0f * ua - 1.0f;
   const float sin_t = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));  // 0 or >= 2^-24
   float sp, cp;
   p_sincos_word(wb, sp, cp);
@@ -610,11 +616,10 @@ __device__ __forceinline__ void emit_genstep(const DevParams& P, const DevGenste
     t0 = G.t0 + u * (G.t1 - G.t0);
     rng.pos += 1u;
   }
-  const float ua = to_unit(rng.template peek<0>());
+  const float cos_t = sym_unit(rng.template peek<0>());
   const uint32_t wb = rng.template peek<1>();
   const float ut = to_unit(rng.template peek<2>());
   rng.pos += 3u;
-  const float cos_t = 2.0f * ua - 1.0f;
   const float sin_t = sqrt_n(__builtin_fmaxf(0.0f, 1.0f - cos_t * cos_t));  // 0 or >= 2^-24
   float sp, cp;
   p_sincos_word(wb, sp, cp);

@@ -141,3 +141,17 @@ def test_tiny_component_test_from_bits():
     got = ((b << 1) - 2) % (1 << 32) < ((0x21800000 << 1) - 2)
     want = (np.abs(vals) > 0) & (np.abs(vals) < edge)
     assert np.array_equal(got, want)
+
+
+def test_sym_unit_equals_two_u_minus_one_for_every_word():
+    """One FMA of k 2^-23 and 2^-23 - 1 gives the reference's 2 u - 1 (u the
+    uniform of word k, transport.hpp:55-67) for every 24-bit k (photon.hpp:
+    sym_unit). The FMA's single rounding of the exact value is taken in
+    float64, where k 2^-23 + (2^-23 - 1) is exact."""
+    k = np.arange(0, 1 << 24, dtype=np.uint32)
+    u = (k.astype(F) * F(2.0 ** -24) + F(2.0 ** -24)).astype(F)
+    ref = (F(2) * u - F(1)).astype(F)
+    got = (k.astype(np.float64) * 2.0 ** -23 + (2.0 ** -23 - 1.0)).astype(F)
+    assert np.array_equal(ref.view(np.uint32), got.view(np.uint32))
+    # the constant itself is exact in float32
+    assert float(F(2.0 ** -23 - 1.0)) == 2.0 ** -23 - 1.0
