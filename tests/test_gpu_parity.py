@@ -148,6 +148,9 @@ def test_tally_equals_oracle(backend, oracle_portable, name):
         steps = ref["boundary_sum"] + ref["scatter_sum"] + int(ref["fate_counts"][1])
         assert cnt.as_dict()["useful_lane_steps"] == steps
         assert cnt.as_dict()["emitted"] == n
+        # the launch that is timed: the same kernel without the branch counters
+        t2, _, _, _ = backend.run_gpu_tally_wavefront(p, c, n, tuning=tuning, counters=False)
+        np.testing.assert_array_equal(t2.flat(), t.flat())
 
 
 @pytest.mark.parametrize("name,n", [("c1_plain", 1_000_000), ("c2_long", 1_000_000),
