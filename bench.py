@@ -642,18 +642,28 @@ def main():
     if args.valu_probe:
         import ctypes
 
-        from gpu_optical_photons_amd import backend
-        fn = backend.lib().optphot_valu_probe
+        # the calibration kernel lives in a library of its own, beside the
+        # transport library (gpu_optical_photons_amd/csrc/probe.hip)
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_optical_photons_amd",
+                            "liboptphot_probe.so")
+        if not os.path.exists(path):
+            raise SystemExit(f"bench --valu-probe: {path} is not built (make -C "
+                             "gpu_optical_photons_amd/csrc)")
+        fn = ctypes.CDLL(path).ophip_valu_probe
         fn.restype = ctypes.c_int
         fn.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                        ctypes.POINTER(ctypes.c_double)]
+
+        def check(rc):
+            if rc != 0:
+                raise RuntimeError(f"valu probe failed ({rc})")
         blocks, iters = 2048, 1 << 15
         out = {"blocks": blocks, "threads_per_block": 256, "iters": iters,
                "fma_per_wave": 8 * iters}
         for half in (0, 1):
             ms = ctypes.c_double()
-            backend._check(fn(blocks, iters, half, ctypes.byref(ms)))  # warm-up
-            backend._check(fn(blocks, iters, half, ctypes.byref(ms)))
+            check(fn(blocks, iters, half, ctypes.byref(ms)))  # warm-up
+            check(fn(blocks, iters, half, ctypes.byref(ms)))
             waves = blocks * 4
             key = "half_lanes" if half else "all_lanes"
             out[key] = {"kernel_ms": ms.value,

@@ -912,24 +912,6 @@ __attribute__((visibility("default"))) int optphot_device_info(char* buf, size_t
   });
 }
 
-__attribute__((visibility("default"))) int optphot_valu_probe(uint32_t blocks, uint32_t iters,
-                                                              int half_lanes, double* kernel_ms) {
-  return guarded([&] {
-    if (blocks == 0 || iters == 0) throw InvalidArgument("valu probe: blocks and iters >= 1");
-    Context& c = ctx();
-    std::lock_guard<std::mutex> lock(c.mu);
-    c.init();
-    // the probe's output lives in the partial-histogram buffer
-    c.partials.ensure(static_cast<size_t>(blocks) * 256 * sizeof(float));
-    HIPCHK(hipEventRecord(c.ev[0], c.stream));
-    launch_valu_probe(c.partials.as<float>(), blocks, iters, half_lanes != 0, c.stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c.ev[1], c.stream));
-    const float ms = c.elapsed(0, 1);
-    if (kernel_ms) *kernel_ms = ms;
-  });
-}
-
 __attribute__((visibility("default"))) int optphot_run_gpu_records(
     const optphot_sim_params* p, uint64_t n, const optphot_records* out, int block_size,
     uint64_t first_id, uint64_t chunk, optphot_run_timing* timing) {

@@ -270,11 +270,6 @@ void launch_chi2_weights(const uint32_t* counts, uint32_t words, uint32_t bins1,
                          const double* inv_nt, double* hist, double* loss, double* weights,
                          uint32_t n_sets, hipStream_t stream);
 
-// VALU calibration kernel: `blocks` x 256 lanes, each iters x 8 FMAs (only odd
-// lanes when `half`); out holds blocks x 256 floats.
-void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
-                       hipStream_t stream);
-
 // Occupancy queries (blocks per CU for a block size and dynamic LDS bytes).
 int history_blocks_per_cu(bool tally, int block, size_t lds_bytes);  // all dynamic LDS
 // Words of the LDS ring region of a block: 16 slots of the block size rounded

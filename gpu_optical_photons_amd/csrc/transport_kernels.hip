@@ -1821,31 +1821,6 @@ __global__ __launch_bounds__(256) void chi2_weights_kernel(
   }
 }
 
-// Calibration of the VALU counters and issue rate: 8 independent FMA chains
-// per lane, `iters` x 8 v_fma_f32 per wave (the loop is unrolled by 4; its
-// control is scalar). With `half` only odd lanes run the loop, so a profiler's
-// active-lane figure for this kernel must read 32 of 64.
-__global__ __launch_bounds__(256) void valu_probe_kernel(float* out, uint32_t iters,
-                                                         uint32_t half) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  float a0 = gid * 1e-9f, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f;
-  float a4 = a0 + 4.0f, a5 = a0 + 5.0f, a6 = a0 + 6.0f, a7 = a0 + 7.0f;
-  if (!half || (lane_id() & 1u)) {
-#pragma unroll 4
-    for (uint32_t i = 0; i < iters; ++i) {
-      a0 = __builtin_fmaf(a0, 0.999999f, 1e-7f);
-      a1 = __builtin_fmaf(a1, 0.999999f, 1e-7f);
-      a2 = __builtin_fmaf(a2, 0.999999f, 1e-7f);
-      a3 = __builtin_fmaf(a3, 0.999999f, 1e-7f);
-      a4 = __builtin_fmaf(a4, 0.999999f, 1e-7f);
-      a5 = __builtin_fmaf(a5, 0.999999f, 1e-7f);
-      a6 = __builtin_fmaf(a6, 0.999999f, 1e-7f);
-      a7 = __builtin_fmaf(a7, 0.999999f, 1e-7f);
-    }
-  }
-  out[gid] = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
-}
-
 template <bool R, bool T, bool E, bool C>
 void launch_history_t(const DevParams& P, PhiloxKey key, uint64_t first_id, uint64_t n, int block,
                       int grid, const LaunchOut& out, hipStream_t stream) {
@@ -2202,12 +2177,6 @@ void launch_gensteps_surf(const DevParams& P, PhiloxKey key, uint64_t first_id,
     case kSpecular: launch_gensteps_t<R, C, kSpecular>(P, key, first_id, s, g, queue, out, stream); break;
     default: launch_gensteps_t<R, C, kLambertian>(P, key, first_id, s, g, queue, out, stream); break;
   }
-}
-
-void launch_valu_probe(float* out, uint32_t blocks, uint32_t iters, bool half,
-                       hipStream_t stream) {
-  hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, stream, out, iters,
-                     half ? 1u : 0u);
 }
 
 void launch_rayleigh_table(float* table, hipStream_t stream) {

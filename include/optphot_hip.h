@@ -393,12 +393,6 @@ int optphot_run_gpu_chi2_step(const optphot_sim_params* params, const uint64_t* 
                               const optphot_wavefront_tuning* adjoint_tuning,
                               optphot_run_timing* timing);
 
-/* Calibration of profiler VALU counters: launches `blocks` x 256 lanes, each
- * running iters x 8 independent v_fma_f32 (only the odd lanes of every wave
- * when half_lanes != 0), and returns the kernel time. A wave executes
- * iters x 8 VALU FMAs plus a handful of setup instructions. */
-int optphot_valu_probe(uint32_t blocks, uint32_t iters, int half_lanes, double* kernel_ms);
-
 /* Copies the device's Rayleigh-cosine table into out (n must be 2^24):
  * entry k is the Rayleigh scattering cosine (sampling.hpp:39-44) of the
  * uniform (k + 1) 2^-24, i.e. of every Philox word whose top 24 bits are k.
