@@ -324,9 +324,9 @@ __device__ __forceinline__ float p_log(float x) {
 // r = v - q, and r > 1/2 becomes r - 1 with q + 1; then x = r (pi / 2). Every
 // step before the product is exact: v = (k + 1) 2^-22, so q and r are the
 // quotient and remainder of k + 1 by 2^22, rounded to nearest with ties down
-// (r = 1/2 stays). With t = k + 2^21: q = t >> 22 and r = ((t & (2^22 - 1))
-// - (2^21 - 1)) 2^-22, an integer in (-2^21, 2^21] times 2^-22, and
-// x = r' (2^-22 pi/2) rounds as r (pi/2) does (a power-of-two scale commutes
+// (r = 1/2 stays). With t = k + 2^21: q = t >> 22 and r = ri 2^-22 with the
+// integer ri = (t & (2^22 - 1)) - (2^21 - 1) in (-2^21, 2^21], and
+// x = ri (2^-22 pi/2) rounds as r (pi/2) does (a power-of-two scale commutes
 // with rounding, and x is 0 or normal). The polynomials are unchanged.
 __device__ __forceinline__ void p_sincos_word(uint32_t w, float& sn, float& cs) {
   const uint32_t t = (w >> 8) + (1u << 21);
